@@ -1,0 +1,175 @@
+/*
+ * mvbt.h - C ABI of the MI355X-native multiversion B-link tree.
+ *
+ * One handle owns one tree (plain B-link or multiversion) on one HIP device:
+ * its node pool, counters, free stack and retire ring. The caller owns every
+ * I/O buffer; buffers are device memory (hipMalloc) or pinned host memory
+ * (hipHostMalloc), like the reference's tests. All batch calls are
+ * asynchronous on the given stream (`void*` = hipStream_t, NULL = default
+ * stream) unless documented otherwise, and never allocate device memory.
+ *
+ * Each entry point names the reference interface it replaces
+ * (owensgroup/MVGpuBTree, paths under include/).
+ *
+ * Keys and values are uint32. Key 0 and key 0xFFFFFFFF are reserved (the
+ * reference uses them as its tombstone and empty sentinel): insert/erase of a
+ * reserved key is skipped and raises MVBT_FLAG_INVALID_KEY; find returns
+ * MVBT_NOT_FOUND for it. A find miss returns MVBT_NOT_FOUND (0xFFFFFFFF), as in
+ * the reference (versioned_node.hpp:267-271).
+ */
+#ifndef MVBT_H_
+#define MVBT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MVBT_NOT_FOUND 0xFFFFFFFFu
+#define MVBT_LATEST 0xFFFFFFFFu /* snapshot id meaning "latest version" */
+
+typedef enum mvbt_status {
+  MVBT_OK = 0,
+  MVBT_ERR_INVALID_ARG = 1,  /* null handle/pointer, bad config, n too large */
+  MVBT_ERR_OUT_OF_MEMORY = 2,/* device allocation failed at create */
+  MVBT_ERR_HIP = 3,          /* a HIP runtime call failed; see mvbt_last_error() */
+  MVBT_ERR_UNSUPPORTED = 4,  /* operation not available for this tree kind */
+  MVBT_ERR_DEVICE_FLAGS = 5  /* device error flags are set; see mvbt_error_flags() */
+} mvbt_status;
+
+typedef enum mvbt_kind {
+  MVBT_PLAIN = 0,    /* GpuBTree::gpu_blink_tree        (gpu_blink_tree.hpp:63-1194) */
+  MVBT_VERSIONED = 1 /* GpuBTree::gpu_versioned_btree   (gpu_versioned_blink_tree.hpp:71-1719) */
+} mvbt_kind;
+
+/* Device error flags (sticky until mvbt_clear_error_flags). */
+#define MVBT_FLAG_POOL_EXHAUSTED 0x1u  /* node pool full: some writes were dropped */
+#define MVBT_FLAG_RETIRE_OVERFLOW 0x2u /* retire ring full: some copies are never freed */
+#define MVBT_FLAG_INVALID_KEY 0x4u     /* reserved key passed to insert/erase */
+#define MVBT_FLAG_VERSION_GAP 0x8u     /* read at a snapshot older than the data */
+#define MVBT_FLAG_RANGE_OVERFLOW 0x10u /* a range query matched more than L pairs */
+#define MVBT_FLAG_CORRUPT 0x20u        /* a traversal left the pool or hit its step bound */
+
+/* Operation flags. */
+#define MVBT_CONCURRENT 0x1u /* readers may run while writers run on another stream
+                                (reference `bool concurrent`): use coherent loads and
+                                validate leaf versions */
+
+typedef struct mvbt_config {
+  uint32_t kind;            /* mvbt_kind */
+  uint32_t node_width;      /* pairs per node: 16 (128 B), 32 or 64; 0 = 16 */
+  uint64_t pool_nodes;      /* node slots (< 2^30); 0 = 1<<22 */
+  uint64_t retire_capacity; /* retire ring entries (power of two); 0 = 1<<22 */
+  int32_t device;           /* HIP device ordinal; -1 = current device */
+  uint32_t reserved[3];
+} mvbt_config;
+
+/* {key, value}; same 8-byte layout as the reference pair_type
+ * (pair_type.hpp:21-64). */
+typedef struct mvbt_pair {
+  uint32_t first;
+  uint32_t second;
+} mvbt_pair;
+
+typedef struct mvbt_memstats {
+  uint64_t node_bytes;      /* bytes per node */
+  uint64_t capacity_nodes;  /* pool slots */
+  uint64_t live_nodes;      /* slots in use (tree nodes + unreclaimed version copies) */
+  uint64_t peak_live_nodes; /* max live_nodes seen at reclaim/stat points */
+  uint64_t free_nodes;      /* slots available to allocation */
+  uint64_t retired_pending; /* retired copies not yet reclaimed */
+  uint64_t reclaimed_nodes; /* total slots returned by reclamation */
+  uint64_t retired_total;   /* total copies ever retired */
+} mvbt_memstats;
+
+typedef struct mvbt_tree mvbt_tree;
+
+/* Human-readable text of the last error on the calling thread. */
+const char* mvbt_last_error(void);
+
+/* Replaces the constructors gpu_versioned_btree() (gpu_versioned_blink_tree.hpp:88-93)
+ * and gpu_blink_tree() (gpu_blink_tree.hpp, initialize_kernel btree_kernels.hpp:525-534).
+ * Synchronous. */
+mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out);
+/* Replaces the destructors (shared_ptr + cuda_deleter, host_allocators.hpp:19-22).
+ * Synchronous (waits for the device). */
+mvbt_status mvbt_destroy(mvbt_tree* tree);
+
+/* Replaces gpu_versioned_btree::insert(keys, values, n, stream, in_place=true)
+ * (gpu_versioned_blink_tree.hpp:106-142) and gpu_blink_tree::insert
+ * (gpu_blink_tree.hpp:218-225). An existing key gets the new value. */
+mvbt_status mvbt_insert(mvbt_tree* tree, const uint32_t* keys, const uint32_t* values,
+                        uint64_t n, void* stream, uint32_t flags);
+
+/* Replaces find(keys, values, n, [timestamp,] stream, concurrent)
+ * (gpu_versioned_blink_tree.hpp:144-164, gpu_blink_tree.hpp:227-236).
+ * values[i] = value of keys[i] visible at `snapshot` (MVBT_LATEST for the
+ * latest version; plain trees accept only MVBT_LATEST), else MVBT_NOT_FOUND. */
+mvbt_status mvbt_find(mvbt_tree* tree, const uint32_t* keys, uint32_t* values, uint64_t n,
+                      uint32_t snapshot, void* stream, uint32_t flags);
+
+/* Replaces erase(keys, n, stream, concurrent) (gpu_versioned_blink_tree.hpp:166-173,
+ * gpu_blink_tree.hpp:238-245). On a versioned tree the erase is versioned:
+ * snapshots taken before it still see the key (the reference per-batch erase is
+ * not; its versioned erase exists only inside concurrent_find_erase). */
+mvbt_status mvbt_erase(mvbt_tree* tree, const uint32_t* keys, uint64_t n, void* stream,
+                       uint32_t flags);
+
+/* Replaces range_query(lower, upper, result, counts, average_range_length, n,
+ * [timestamp,] stream, concurrent) (gpu_versioned_blink_tree.hpp:175-216,
+ * gpu_blink_tree.hpp:247-265). Query i covers [lower[i], upper[i]); its pairs,
+ * ascending, go to out[i*L ...]; at most L are written (the reference writes
+ * past the slot) and counts[i] receives the true count. `out` may be NULL to
+ * count only; `counts` may be NULL. */
+mvbt_status mvbt_range_query(mvbt_tree* tree, const uint32_t* lower, const uint32_t* upper,
+                             mvbt_pair* out, uint32_t* counts, uint32_t L, uint64_t n,
+                             uint32_t snapshot, void* stream, uint32_t flags);
+
+/* Replaces take_snapshot(stream) (gpu_versioned_blink_tree.hpp:297-302): returns
+ * the id s; writes issued after it are not visible at s. Blocks until the
+ * snapshot kernel has run, like the reference. The snapshot stays live (its
+ * versions are kept) until mvbt_release_snapshot. Versioned trees only. */
+mvbt_status mvbt_take_snapshot(mvbt_tree* tree, void* stream, uint32_t* snapshot_id);
+
+/* No reference counterpart (the reference never frees in-place version copies):
+ * marks snapshot s as no longer read. */
+mvbt_status mvbt_release_snapshot(mvbt_tree* tree, uint32_t snapshot_id);
+
+/* Epoch reclamation (reference DEBR, memory_reclaimer.hpp:244-401, driven here
+ * by snapshot lifetimes): returns to the free list every version copy that no
+ * live snapshot can read. Stream-ordered; must not overlap writers or readers
+ * of older snapshots on other streams. */
+mvbt_status mvbt_reclaim(mvbt_tree* tree, void* stream);
+
+/* Memory accounting (reference compute_memory_usage gpu_versioned_blink_tree.hpp:1442-1446,
+ * COLLECT_ALLOCATOR_STATS slab_alloc.hpp:489-540). Synchronous. */
+mvbt_status mvbt_memory_stats(mvbt_tree* tree, mvbt_memstats* out);
+
+/* Sticky device error flags (MVBT_FLAG_*). Synchronous. */
+mvbt_status mvbt_error_flags(mvbt_tree* tree, uint32_t* flags);
+mvbt_status mvbt_clear_error_flags(mvbt_tree* tree);
+
+/* Replaces validate_tree_structure (gpu_versioned_blink_tree.hpp:1098-1259,
+ * gpu_blink_tree.hpp:743-900) for every allocator: copies the pool to the host
+ * and checks that no node is locked, keys are sorted and below each high key,
+ * every level's sibling chain is ordered, and leaves hold exactly the latest
+ * key set. Writes the number of latest keys to *num_keys and the tree height
+ * to *height (either may be NULL). Synchronous. Returns MVBT_ERR_DEVICE_FLAGS
+ * with a message when an invariant fails. */
+mvbt_status mvbt_validate(mvbt_tree* tree, uint64_t* num_keys, uint32_t* height);
+
+/* Export of the latest (key, value) pairs in key order (reference
+ * copy_tree_to_host, gpu_versioned_blink_tree.hpp:1092-1094). `out` holds
+ * `capacity` pairs; *count receives the number of keys. Synchronous. */
+mvbt_status mvbt_export(mvbt_tree* tree, mvbt_pair* out, uint64_t capacity, uint64_t* count);
+
+/* Tree parameters. */
+mvbt_status mvbt_node_width(mvbt_tree* tree, uint32_t* node_width);
+mvbt_status mvbt_kind_of(mvbt_tree* tree, uint32_t* kind);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MVBT_H_ */

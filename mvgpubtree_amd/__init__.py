@@ -1,0 +1,295 @@
+"""MI355X-native multiversion B-link tree (host mirror over the C ABI).
+
+The product is ``libmvbt.so`` (HIP kernels for gfx950 + the C ABI declared in
+``include/mvbt.h``). This module binds that ABI with ctypes and mirrors the
+reference's host API (owensgroup/MVGpuBTree ``gpu_versioned_btree`` and
+``gpu_blink_tree``, include/gpu_versioned_blink_tree.hpp:106-302 and
+include/gpu_blink_tree.hpp:218-265): ``insert``, ``find``, ``erase``,
+``range_query`` and ``take_snapshot`` with the same argument meaning. PyTorch
+only provides device memory and streams: every array argument is a device
+tensor with 4-byte elements (``torch.int32`` / ``torch.uint32``), or 8-byte
+elements for range-query pairs.
+
+Loading fails loudly when ``libmvbt.so`` is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+__all__ = [
+    "GpuBlinkTree", "GpuVersionedBTree", "MvbtError", "LATEST", "NOT_FOUND",
+    "FLAG_POOL_EXHAUSTED", "FLAG_RETIRE_OVERFLOW", "FLAG_INVALID_KEY", "FLAG_VERSION_GAP",
+    "FLAG_RANGE_OVERFLOW", "FLAG_CORRUPT",
+    "CONCURRENT", "library_path", "load_library", "EXPORTED_SYMBOLS",
+]
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmvbt.so")
+
+LATEST = 0xFFFFFFFF
+NOT_FOUND = 0xFFFFFFFF
+CONCURRENT = 0x1
+FLAG_POOL_EXHAUSTED = 0x1
+FLAG_RETIRE_OVERFLOW = 0x2
+FLAG_INVALID_KEY = 0x4
+FLAG_VERSION_GAP = 0x8
+FLAG_RANGE_OVERFLOW = 0x10
+FLAG_CORRUPT = 0x20
+PLAIN, VERSIONED = 0, 1
+
+_STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "UNSUPPORTED", 5: "DEVICE_FLAGS"}
+
+# Every symbol declared in include/mvbt.h.
+EXPORTED_SYMBOLS = (
+    "mvbt_last_error", "mvbt_create", "mvbt_destroy", "mvbt_insert", "mvbt_find", "mvbt_erase",
+    "mvbt_range_query", "mvbt_take_snapshot", "mvbt_release_snapshot", "mvbt_reclaim",
+    "mvbt_memory_stats", "mvbt_error_flags", "mvbt_clear_error_flags", "mvbt_validate",
+    "mvbt_export", "mvbt_node_width", "mvbt_kind_of",
+)
+
+
+class MvbtError(RuntimeError):
+    def __init__(self, status: int, op: str, msg: str):
+        super().__init__(f"{op}: {_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("node_width", ctypes.c_uint32),
+                ("pool_nodes", ctypes.c_uint64), ("retire_capacity", ctypes.c_uint64),
+                ("device", ctypes.c_int32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class _MemStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "node_bytes", "capacity_nodes", "live_nodes", "peak_live_nodes", "free_nodes",
+        "retired_pending", "reclaimed_nodes", "retired_total")]
+
+
+@dataclass
+class MemoryStats:
+    node_bytes: int
+    capacity_nodes: int
+    live_nodes: int
+    peak_live_nodes: int
+    free_nodes: int
+    retired_pending: int
+    reclaimed_nodes: int
+    retired_total: int
+
+    @property
+    def live_bytes(self) -> int:
+        return self.live_nodes * self.node_bytes
+
+    @property
+    def peak_bytes(self) -> int:
+        return self.peak_live_nodes * self.node_bytes
+
+    @property
+    def free_bytes(self) -> int:
+        return self.free_nodes * self.node_bytes
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load_library():
+    """Loads libmvbt.so and declares its C ABI. Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                          "(python -m mvgpubtree_amd.build)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    u32p, u64p = ctypes.POINTER(u32), ctypes.POINTER(u64)
+    sig = {
+        "mvbt_last_error": ([], ctypes.c_char_p),
+        "mvbt_create": ([ctypes.POINTER(_Config), ctypes.POINTER(vp)], i32),
+        "mvbt_destroy": ([vp], i32),
+        "mvbt_insert": ([vp, vp, vp, u64, vp, u32], i32),
+        "mvbt_find": ([vp, vp, vp, u64, u32, vp, u32], i32),
+        "mvbt_erase": ([vp, vp, u64, vp, u32], i32),
+        "mvbt_range_query": ([vp, vp, vp, vp, vp, u32, u64, u32, vp, u32], i32),
+        "mvbt_take_snapshot": ([vp, vp, u32p], i32),
+        "mvbt_release_snapshot": ([vp, u32], i32),
+        "mvbt_reclaim": ([vp, vp], i32),
+        "mvbt_memory_stats": ([vp, ctypes.POINTER(_MemStats)], i32),
+        "mvbt_error_flags": ([vp, u32p], i32),
+        "mvbt_clear_error_flags": ([vp], i32),
+        "mvbt_validate": ([vp, u64p, u32p], i32),
+        "mvbt_export": ([vp, vp, u64, u64p], i32),
+        "mvbt_node_width": ([vp, u32p], i32),
+        "mvbt_kind_of": ([vp, u32p], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(status: int, op: str):
+    if status != 0:
+        msg = load_library().mvbt_last_error()
+        raise MvbtError(status, op, msg.decode() if msg else "")
+
+
+def _torch():
+    import torch  # device memory and streams only
+    return torch
+
+
+def _ptr(t, elem_bytes: int, name: str):
+    if t is None:
+        return None
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor on the GPU")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must live in device memory")
+    if t.element_size() != elem_bytes or not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous with {elem_bytes}-byte elements")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        torch = _torch()
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class _TreeBase:
+    _kind = PLAIN
+
+    def __init__(self, pool_nodes: int = 1 << 22, node_width: int = 16,
+                 retire_capacity: int = 1 << 22, device: int = -1):
+        L = load_library()
+        cfg = _Config(kind=self._kind, node_width=node_width, pool_nodes=pool_nodes,
+                      retire_capacity=retire_capacity, device=device)
+        h = ctypes.c_void_p()
+        _check(L.mvbt_create(ctypes.byref(cfg), ctypes.byref(h)), "mvbt_create")
+        self._h = h
+        self.node_width = node_width
+        self.branching_factor = node_width
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().mvbt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # --- batch API (reference host methods) -------------------------------
+    def insert(self, keys, values, stream=None):
+        n = keys.numel()
+        if values.numel() != n:
+            raise ValueError("keys and values differ in length")
+        _check(load_library().mvbt_insert(self._h, _ptr(keys, 4, "keys"), _ptr(values, 4, "values"),
+                                          n, _stream(stream), 0), "insert")
+
+    def erase(self, keys, stream=None, concurrent: bool = False):
+        _check(load_library().mvbt_erase(self._h, _ptr(keys, 4, "keys"), keys.numel(),
+                                         _stream(stream), CONCURRENT if concurrent else 0), "erase")
+
+    def find(self, keys, values=None, timestamp: int = LATEST, stream=None, concurrent: bool = False):
+        torch = _torch()
+        if values is None:
+            values = torch.empty(keys.numel(), dtype=torch.int32, device=keys.device)
+        if values.numel() < keys.numel():
+            raise ValueError("values shorter than keys")
+        _check(load_library().mvbt_find(self._h, _ptr(keys, 4, "keys"), _ptr(values, 4, "values"),
+                                        keys.numel(), timestamp, _stream(stream),
+                                        CONCURRENT if concurrent else 0), "find")
+        return values
+
+    def range_query(self, lower, upper, average_range_length: int, result=None, counts=None,
+                    timestamp: int = LATEST, stream=None, concurrent: bool = False,
+                    want_pairs: bool = True):
+        """Returns (pairs int64[n*L] with key in the low word, counts int32[n])."""
+        torch = _torch()
+        n, L = lower.numel(), average_range_length
+        if upper.numel() != n:
+            raise ValueError("lower and upper differ in length")
+        if result is None and want_pairs:
+            result = torch.full((n * L,), -1, dtype=torch.int64, device=lower.device)
+        if counts is None:
+            counts = torch.empty(n, dtype=torch.int32, device=lower.device)
+        if result is not None and result.numel() < n * L:
+            raise ValueError("result shorter than n * average_range_length")
+        _check(load_library().mvbt_range_query(
+            self._h, _ptr(lower, 4, "lower"), _ptr(upper, 4, "upper"), _ptr(result, 8, "result"),
+            _ptr(counts, 4, "counts"), L, n, timestamp, _stream(stream),
+            CONCURRENT if concurrent else 0), "range_query")
+        return result, counts
+
+    # --- maintenance -----------------------------------------------------------
+    def memory_stats(self) -> MemoryStats:
+        s = _MemStats()
+        _check(load_library().mvbt_memory_stats(self._h, ctypes.byref(s)), "memory_stats")
+        return MemoryStats(**{f: getattr(s, f) for f, _ in _MemStats._fields_})
+
+    def error_flags(self) -> int:
+        f = ctypes.c_uint32()
+        _check(load_library().mvbt_error_flags(self._h, ctypes.byref(f)), "error_flags")
+        return f.value
+
+    def clear_error_flags(self):
+        _check(load_library().mvbt_clear_error_flags(self._h), "clear_error_flags")
+
+    def validate_tree_structure(self):
+        """Returns (num_keys, height); raises MvbtError when an invariant fails."""
+        nk, h = ctypes.c_uint64(), ctypes.c_uint32()
+        _check(load_library().mvbt_validate(self._h, ctypes.byref(nk), ctypes.byref(h)), "validate")
+        return nk.value, h.value
+
+    def export(self):
+        """Latest pairs in key order as a numpy uint64 array (key in the low word)."""
+        import numpy as np
+        cnt = ctypes.c_uint64()
+        _check(load_library().mvbt_export(self._h, None, 0, ctypes.byref(cnt)), "export")
+        out = np.empty(cnt.value, dtype=np.uint64)
+        _check(load_library().mvbt_export(self._h, ctypes.c_void_p(out.ctypes.data), out.size,
+                                          ctypes.byref(cnt)), "export")
+        return out[: cnt.value]
+
+
+class GpuBlinkTree(_TreeBase):
+    """Plain concurrent B-link tree (reference ``gpu_blink_tree``)."""
+    _kind = PLAIN
+
+
+class GpuVersionedBTree(_TreeBase):
+    """Multiversion B-link tree (reference ``gpu_versioned_btree``)."""
+    _kind = VERSIONED
+
+    def take_snapshot(self, stream=None) -> int:
+        sid = ctypes.c_uint32()
+        _check(load_library().mvbt_take_snapshot(self._h, _stream(stream), ctypes.byref(sid)),
+               "take_snapshot")
+        return sid.value
+
+    def release_snapshot(self, snapshot_id: int):
+        _check(load_library().mvbt_release_snapshot(self._h, snapshot_id), "release_snapshot")
+
+    def reclaim(self, stream=None):
+        _check(load_library().mvbt_reclaim(self._h, _stream(stream)), "reclaim")

@@ -1,0 +1,540 @@
+// C ABI implementation (include/mvbt.h): handle management, launch sizing,
+// snapshot bookkeeping, epoch reclamation and host-side validation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mvbt.h"
+#include "mvbt_kernels.hpp"
+
+using namespace mvbt;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+mvbt_status fail(mvbt_status s, const std::string& msg) {
+  g_last_error = msg;
+  return s;
+}
+
+#define MVBT_HIP(call)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(MVBT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+constexpr int kBlock = 256;
+
+}  // namespace
+
+struct mvbt_tree {
+  mvbt_config cfg{};
+  int device = 0;
+  uint32_t nw = 16;
+  bool versioned = false;
+  int num_cus = 256;
+  uint64_t* nodes = nullptr;
+  tree_state* st = nullptr;
+  uint32_t* free_stack = nullptr;
+  uint32_t* retire_ring = nullptr;
+  snapshot_record* snap_rec = nullptr;  // pinned host slot
+  std::mutex mu;
+  std::map<uint32_t, uint32_t> live;                  // live snapshot id -> refcount
+  std::vector<std::pair<uint32_t, uint64_t>> bounds;  // (snapshot id, ring pos at that epoch edge)
+
+  template <int NW, bool VER>
+  tree_view<NW, VER> view() const {
+    tree_view<NW, VER> v;
+    v.nodes = nodes;
+    v.st = st;
+    v.free_stack = free_stack;
+    v.retire_ring = retire_ring;
+    v.root_ = 0;
+    v.cap_ = (uint32_t)cfg.pool_nodes;
+    return v;
+  }
+};
+
+namespace {
+
+// RAII device switch.
+struct device_guard {
+  int prev = -1;
+  bool ok = true;
+  explicit device_guard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~device_guard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+uint32_t grid_for(const mvbt_tree* t, uint64_t n, uint32_t nw, int waves_per_cu) {
+  const uint64_t ops_per_block = kBlock;  // one op per lane per chunk
+  uint64_t need = (n + ops_per_block - 1) / ops_per_block;
+  uint64_t cap = (uint64_t)t->num_cus * (uint64_t)waves_per_cu * 64 / kBlock;
+  (void)nw;
+  if (need < 1) need = 1;
+  return (uint32_t)std::min<uint64_t>(need, std::max<uint64_t>(cap, 1));
+}
+
+template <int NW, bool VER>
+struct impl {
+  static mvbt_status init(mvbt_tree* t) {
+    hipLaunchKernelGGL((init_kernel<NW, VER>), dim3(1), dim3(64), 0, 0, t->template view<NW, VER>());
+    MVBT_HIP(hipGetLastError());
+    return MVBT_OK;
+  }
+  static mvbt_status find(mvbt_tree* t, const uint32_t* keys, uint32_t* vals, uint64_t n,
+                          uint32_t ts, hipStream_t s, bool conc) {
+    uint32_t g = grid_for(t, n, NW, 32);
+    if (conc)
+      hipLaunchKernelGGL((find_kernel<NW, VER, true>), dim3(g), dim3(kBlock), 0, s,
+                         t->template view<NW, VER>(), keys, vals, n, ts);
+    else
+      hipLaunchKernelGGL((find_kernel<NW, VER, false>), dim3(g), dim3(kBlock), 0, s,
+                         t->template view<NW, VER>(), keys, vals, n, ts);
+    MVBT_HIP(hipGetLastError());
+    return MVBT_OK;
+  }
+  static mvbt_status range(mvbt_tree* t, const uint32_t* lo, const uint32_t* hi, uint64_t* out,
+                           uint32_t* counts, uint32_t L, uint64_t n, uint32_t ts, hipStream_t s,
+                           bool conc) {
+    uint32_t g = grid_for(t, n, NW, 32);
+    if (conc)
+      hipLaunchKernelGGL((range_kernel<NW, VER, true>), dim3(g), dim3(kBlock), 0, s,
+                         t->template view<NW, VER>(), lo, hi, out, counts, L, n, ts);
+    else
+      hipLaunchKernelGGL((range_kernel<NW, VER, false>), dim3(g), dim3(kBlock), 0, s,
+                         t->template view<NW, VER>(), lo, hi, out, counts, L, n, ts);
+    MVBT_HIP(hipGetLastError());
+    return MVBT_OK;
+  }
+  template <int OP>
+  static mvbt_status write(mvbt_tree* t, const uint32_t* keys, const uint32_t* vals, uint64_t n,
+                           hipStream_t s) {
+    uint32_t g = grid_for(t, n, NW, 32);
+    hipLaunchKernelGGL((write_kernel<NW, VER, OP>), dim3(g), dim3(kBlock), 0, s,
+                       t->template view<NW, VER>(), keys, vals, n);
+    MVBT_HIP(hipGetLastError());
+    return MVBT_OK;
+  }
+};
+
+struct dispatch_table {
+  mvbt_status (*init)(mvbt_tree*);
+  mvbt_status (*find)(mvbt_tree*, const uint32_t*, uint32_t*, uint64_t, uint32_t, hipStream_t, bool);
+  mvbt_status (*range)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t*, uint32_t*, uint32_t,
+                       uint64_t, uint32_t, hipStream_t, bool);
+  mvbt_status (*insert)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, hipStream_t);
+  mvbt_status (*erase)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, hipStream_t);
+};
+
+template <int NW, bool VER>
+dispatch_table make_table() {
+  using I = impl<NW, VER>;
+  return dispatch_table{&I::init, &I::find, &I::range, &I::template write<0>, &I::template write<1>};
+}
+
+const dispatch_table* table_for(uint32_t nw, bool versioned) {
+  static const dispatch_table t16p = make_table<16, false>();
+  static const dispatch_table t16v = make_table<16, true>();
+  static const dispatch_table t32p = make_table<32, false>();
+  static const dispatch_table t32v = make_table<32, true>();
+  static const dispatch_table t64p = make_table<64, false>();
+  static const dispatch_table t64v = make_table<64, true>();
+  switch (nw) {
+    case 16: return versioned ? &t16v : &t16p;
+    case 32: return versioned ? &t32v : &t32p;
+    case 64: return versioned ? &t64v : &t64p;
+    default: return nullptr;
+  }
+}
+
+bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+
+mvbt_status check_tree(mvbt_tree* t) {
+  if (!t) return fail(MVBT_ERR_INVALID_ARG, "null tree handle");
+  return MVBT_OK;
+}
+
+struct host_state_view {
+  tree_state s;
+};
+
+mvbt_status read_state(mvbt_tree* t, tree_state* out) {
+  MVBT_HIP(hipDeviceSynchronize());
+  MVBT_HIP(hipMemcpy(out, t->st, sizeof(tree_state), hipMemcpyDeviceToHost));
+  return MVBT_OK;
+}
+
+// Host copy of the node pool plus a walk over the latest tree.
+struct host_walk {
+  uint32_t nw = 16;
+  bool versioned = false;
+  std::vector<uint64_t> pool;
+  uint64_t nodes = 0;
+  uint32_t data_lanes() const { return versioned ? nw - 2 : nw - 1; }
+  const uint64_t* at(uint32_t i) const { return pool.data() + (size_t)i * nw; }
+  uint64_t meta(uint32_t i) const { return at(i)[nw - 1]; }
+  static uint32_t key(uint64_t p) { return (uint32_t)p; }
+  static uint32_t val(uint64_t p) { return (uint32_t)(p >> 32); }
+};
+
+mvbt_status copy_pool(mvbt_tree* t, host_walk& w) {
+  tree_state s;
+  mvbt_status rc = read_state(t, &s);
+  if (rc != MVBT_OK) return rc;
+  w.nw = t->nw;
+  w.versioned = t->versioned;
+  w.nodes = std::min<uint64_t>(s.bump, s.capacity);
+  try {
+    w.pool.resize(w.nodes * t->nw);
+  } catch (const std::bad_alloc&) {
+    return fail(MVBT_ERR_OUT_OF_MEMORY, "host copy of the node pool failed to allocate");
+  }
+  MVBT_HIP(hipMemcpy(w.pool.data(), t->nodes, w.nodes * t->nw * sizeof(uint64_t),
+                     hipMemcpyDeviceToHost));
+  return MVBT_OK;
+}
+
+// Walks the latest tree level by level. Checks invariants (reference
+// validate_tree_structure, gpu_versioned_blink_tree.hpp:1098-1259) and
+// appends leaf pairs to `pairs` when non-null.
+mvbt_status walk_tree(const host_walk& w, std::vector<mvbt_pair>* pairs, uint64_t* num_keys,
+                      uint32_t* height) {
+  auto bad = [](const std::string& m) { return fail(MVBT_ERR_DEVICE_FLAGS, "validate: " + m); };
+  const uint32_t nw = w.nw, nd = w.data_lanes();
+  std::vector<uint32_t> level{0u};  // root
+  uint32_t h = 0;
+  uint64_t keys = 0;
+  while (true) {
+    ++h;
+    if (h > 64) return bad("tree deeper than 64 levels");
+    // Follow the sibling chain from the left-most node and compare with `level`.
+    std::vector<uint32_t> next;
+    bool leaf_level = false;
+    uint32_t prev_high = 0;
+    for (size_t li = 0; li < level.size(); ++li) {
+      uint32_t idx = level[li];
+      if (idx >= w.nodes) return bad("node index out of range: " + std::to_string(idx));
+      uint64_t m = w.meta(idx);
+      uint32_t flags = (uint32_t)(m >> 32);
+      if (flags & kLockBit) return bad("node " + std::to_string(idx) + " is locked");
+      bool leaf = (flags & kLeafBit) != 0;
+      if (li == 0) leaf_level = leaf;
+      else if (leaf != leaf_level) return bad("mixed leaf/internal level");
+      uint32_t high = (uint32_t)m;
+      uint32_t sib = flags & kPtrMask;
+      if (li + 1 < level.size()) {
+        if (sib != level[li + 1]) return bad("sibling chain does not match child order");
+      } else if (sib != kNoSibling || high != kEmpty) {
+        return bad("last node of a level has a sibling or a finite high key");
+      }
+      uint32_t last = 0;
+      bool seen = false, ended = false;
+      for (uint32_t l = 0; l < nd; ++l) {
+        uint64_t p = w.at(idx)[l];
+        uint32_t k = host_walk::key(p);
+        if (k == kEmpty) {
+          ended = true;
+          continue;
+        }
+        if (ended) return bad("hole in node " + std::to_string(idx));
+        if (seen && k <= last) return bad("unsorted node " + std::to_string(idx));
+        if (k >= high) return bad("key >= high key in node " + std::to_string(idx));
+        if (li > 0 && k < prev_high) return bad("key below left neighbour's high key");
+        seen = true;
+        last = k;
+        if (leaf) {
+          ++keys;
+          if (pairs) pairs->push_back(mvbt_pair{k, host_walk::val(p)});
+        } else {
+          next.push_back(host_walk::val(p));
+        }
+      }
+      if (!leaf && !seen) return bad("empty internal node " + std::to_string(idx));
+      prev_high = high;
+    }
+    if (leaf_level) break;
+    level.swap(next);
+  }
+  if (num_keys) *num_keys = keys;
+  if (height) *height = h;
+  (void)nw;
+  return MVBT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mvbt_last_error(void) { return g_last_error.c_str(); }
+
+mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
+  if (!config || !out) return fail(MVBT_ERR_INVALID_ARG, "null config or out pointer");
+  *out = nullptr;
+  mvbt_config cfg = *config;
+  if (cfg.kind != MVBT_PLAIN && cfg.kind != MVBT_VERSIONED)
+    return fail(MVBT_ERR_INVALID_ARG, "unknown tree kind");
+  if (cfg.node_width == 0) cfg.node_width = 16;
+  if (!table_for(cfg.node_width, true)) return fail(MVBT_ERR_INVALID_ARG, "node_width must be 16, 32 or 64");
+  if (cfg.pool_nodes == 0) cfg.pool_nodes = 1ull << 22;
+  if (cfg.pool_nodes < 4 || cfg.pool_nodes >= kMaxNodes)
+    return fail(MVBT_ERR_INVALID_ARG, "pool_nodes must be in [4, 2^30-1)");
+  if (cfg.retire_capacity == 0) cfg.retire_capacity = 1ull << 22;
+  if (!is_pow2(cfg.retire_capacity)) return fail(MVBT_ERR_INVALID_ARG, "retire_capacity must be a power of two");
+
+  int dev = cfg.device;
+  if (dev < 0) MVBT_HIP(hipGetDevice(&dev));
+  device_guard g(dev);
+  if (!g.ok) return fail(MVBT_ERR_HIP, "hipSetDevice failed");
+
+  std::unique_ptr<mvbt_tree> t(new (std::nothrow) mvbt_tree());
+  if (!t) return fail(MVBT_ERR_OUT_OF_MEMORY, "host allocation failed");
+  t->cfg = cfg;
+  t->device = dev;
+  t->nw = cfg.node_width;
+  t->versioned = cfg.kind == MVBT_VERSIONED;
+  hipDeviceProp_t prop;
+  MVBT_HIP(hipGetDeviceProperties(&prop, dev));
+  t->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+
+  auto cleanup = [&]() {
+    if (t->nodes) (void)hipFree(t->nodes);
+    if (t->st) (void)hipFree(t->st);
+    if (t->free_stack) (void)hipFree(t->free_stack);
+    if (t->retire_ring) (void)hipFree(t->retire_ring);
+    if (t->snap_rec) (void)hipHostFree(t->snap_rec);
+  };
+  const size_t node_bytes = (size_t)t->nw * sizeof(uint64_t);
+  if (hipMalloc(&t->nodes, cfg.pool_nodes * node_bytes) != hipSuccess ||
+      hipMalloc(&t->st, sizeof(tree_state)) != hipSuccess ||
+      hipMalloc(&t->free_stack, cfg.pool_nodes * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&t->retire_ring, cfg.retire_capacity * sizeof(uint32_t)) != hipSuccess ||
+      hipHostMalloc(&t->snap_rec, sizeof(snapshot_record), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    cleanup();
+    return fail(MVBT_ERR_OUT_OF_MEMORY, "device allocation failed");
+  }
+  tree_state s{};
+  s.root = 0;
+  s.clock = 0;
+  s.bump = 2;  // root + first leaf
+  s.capacity = cfg.pool_nodes;
+  s.free_top = 0;
+  s.retire_cap = cfg.retire_capacity;
+  s.peak_live = 2;
+  if (hipMemcpy(t->st, &s, sizeof(s), hipMemcpyHostToDevice) != hipSuccess) {
+    cleanup();
+    return fail(MVBT_ERR_HIP, "state upload failed");
+  }
+  mvbt_status rc = table_for(t->nw, t->versioned)->init(t.get());
+  if (rc == MVBT_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(MVBT_ERR_HIP, "init kernel failed");
+  if (rc != MVBT_OK) {
+    cleanup();
+    return rc;
+  }
+  *out = t.release();
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_destroy(mvbt_tree* t) {
+  if (!t) return MVBT_OK;
+  device_guard g(t->device);
+  (void)hipDeviceSynchronize();
+  (void)hipFree(t->nodes);
+  (void)hipFree(t->st);
+  (void)hipFree(t->free_stack);
+  (void)hipFree(t->retire_ring);
+  (void)hipHostFree(t->snap_rec);
+  delete t;
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_insert(mvbt_tree* t, const uint32_t* keys, const uint32_t* values, uint64_t n,
+                        void* stream, uint32_t flags) {
+  (void)flags;
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (n == 0) return MVBT_OK;
+  if (!keys || !values) return fail(MVBT_ERR_INVALID_ARG, "null keys or values");
+  device_guard g(t->device);
+  return table_for(t->nw, t->versioned)->insert(t, keys, values, n, (hipStream_t)stream);
+}
+
+mvbt_status mvbt_erase(mvbt_tree* t, const uint32_t* keys, uint64_t n, void* stream, uint32_t flags) {
+  (void)flags;
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (n == 0) return MVBT_OK;
+  if (!keys) return fail(MVBT_ERR_INVALID_ARG, "null keys");
+  device_guard g(t->device);
+  return table_for(t->nw, t->versioned)->erase(t, keys, nullptr, n, (hipStream_t)stream);
+}
+
+mvbt_status mvbt_find(mvbt_tree* t, const uint32_t* keys, uint32_t* values, uint64_t n,
+                      uint32_t snapshot, void* stream, uint32_t flags) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (n == 0) return MVBT_OK;
+  if (!keys || !values) return fail(MVBT_ERR_INVALID_ARG, "null keys or values");
+  if (!t->versioned && snapshot != MVBT_LATEST)
+    return fail(MVBT_ERR_UNSUPPORTED, "plain trees have no snapshots");
+  device_guard g(t->device);
+  return table_for(t->nw, t->versioned)
+      ->find(t, keys, values, n, snapshot, (hipStream_t)stream, (flags & MVBT_CONCURRENT) != 0);
+}
+
+mvbt_status mvbt_range_query(mvbt_tree* t, const uint32_t* lower, const uint32_t* upper,
+                             mvbt_pair* out, uint32_t* counts, uint32_t L, uint64_t n,
+                             uint32_t snapshot, void* stream, uint32_t flags) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (n == 0) return MVBT_OK;
+  if (!lower || !upper) return fail(MVBT_ERR_INVALID_ARG, "null bounds");
+  if (out && L == 0) return fail(MVBT_ERR_INVALID_ARG, "L must be > 0 when out is given");
+  if (!t->versioned && snapshot != MVBT_LATEST)
+    return fail(MVBT_ERR_UNSUPPORTED, "plain trees have no snapshots");
+  device_guard g(t->device);
+  return table_for(t->nw, t->versioned)
+      ->range(t, lower, upper, reinterpret_cast<uint64_t*>(out), counts, L, n, snapshot,
+              (hipStream_t)stream, (flags & MVBT_CONCURRENT) != 0);
+}
+
+mvbt_status mvbt_take_snapshot(mvbt_tree* t, void* stream, uint32_t* snapshot_id) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!snapshot_id) return fail(MVBT_ERR_INVALID_ARG, "null snapshot_id");
+  if (!t->versioned) return fail(MVBT_ERR_UNSUPPORTED, "plain trees have no snapshots");
+  device_guard g(t->device);
+  std::lock_guard<std::mutex> lk(t->mu);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(64), 0, s, t->st, t->snap_rec);
+  MVBT_HIP(hipGetLastError());
+  MVBT_HIP(hipStreamSynchronize(s));
+  uint32_t id = t->snap_rec->id;
+  if (id == MVBT_LATEST) return fail(MVBT_ERR_UNSUPPORTED, "snapshot ids exhausted");
+  t->live[id] += 1;
+  t->bounds.emplace_back(id, t->snap_rec->ring_pos);
+  *snapshot_id = id;
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_release_snapshot(mvbt_tree* t, uint32_t id) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  std::lock_guard<std::mutex> lk(t->mu);
+  auto it = t->live.find(id);
+  if (it == t->live.end()) return fail(MVBT_ERR_INVALID_ARG, "snapshot is not live");
+  if (--it->second == 0) t->live.erase(it);
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_reclaim(mvbt_tree* t, void* stream) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!t->versioned) return MVBT_OK;
+  device_guard g(t->device);
+  std::lock_guard<std::mutex> lk(t->mu);
+  // Copies retired before the edge of the oldest live snapshot are garbage;
+  // with no live snapshot, every retired copy is.
+  unsigned long long end = ~0ull;
+  if (!t->live.empty()) {
+    uint32_t oldest = t->live.begin()->first;
+    end = 0;
+    for (const auto& b : t->bounds)
+      if (b.first == oldest) end = b.second;
+  }
+  // Forget edges no longer needed.
+  std::vector<std::pair<uint32_t, uint64_t>> keep;
+  for (const auto& b : t->bounds)
+    if (t->live.count(b.first)) keep.push_back(b);
+  t->bounds.swap(keep);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(reclaim_prep_kernel, dim3(1), dim3(64), 0, s, t->st, end);
+  MVBT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(reclaim_copy_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->st,
+                     t->retire_ring, t->free_stack);
+  MVBT_HIP(hipGetLastError());
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_memory_stats(mvbt_tree* t, mvbt_memstats* out) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!out) return fail(MVBT_ERR_INVALID_ARG, "null out");
+  device_guard g(t->device);
+  tree_state s;
+  if (mvbt_status rc = read_state(t, &s)) return rc;
+  uint64_t on_stack = s.free_top > 0 ? (uint64_t)s.free_top : 0;
+  uint64_t bump = std::min<uint64_t>(s.bump, s.capacity);
+  out->node_bytes = (uint64_t)t->nw * 8;
+  out->capacity_nodes = s.capacity;
+  out->live_nodes = bump - on_stack;
+  out->peak_live_nodes = std::max<uint64_t>(s.peak_live, out->live_nodes);
+  out->free_nodes = s.capacity - bump + on_stack;
+  out->retired_pending = s.retire_tail - s.retire_head;
+  out->reclaimed_nodes = s.reclaimed;
+  out->retired_total = s.retire_tail;
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_error_flags(mvbt_tree* t, uint32_t* flags) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!flags) return fail(MVBT_ERR_INVALID_ARG, "null flags");
+  device_guard g(t->device);
+  tree_state s;
+  if (mvbt_status rc = read_state(t, &s)) return rc;
+  *flags = s.error_flags;
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_clear_error_flags(mvbt_tree* t) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  device_guard g(t->device);
+  MVBT_HIP(hipDeviceSynchronize());
+  uint32_t zero = 0;
+  MVBT_HIP(hipMemcpy(&t->st->error_flags, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_validate(mvbt_tree* t, uint64_t* num_keys, uint32_t* height) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  device_guard g(t->device);
+  host_walk w;
+  if (mvbt_status rc = copy_pool(t, w)) return rc;
+  return walk_tree(w, nullptr, num_keys, height);
+}
+
+mvbt_status mvbt_export(mvbt_tree* t, mvbt_pair* out, uint64_t capacity, uint64_t* count) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!count) return fail(MVBT_ERR_INVALID_ARG, "null count");
+  device_guard g(t->device);
+  host_walk w;
+  if (mvbt_status rc = copy_pool(t, w)) return rc;
+  std::vector<mvbt_pair> pairs;
+  if (mvbt_status rc = walk_tree(w, &pairs, nullptr, nullptr)) return rc;
+  *count = pairs.size();
+  if (out) std::memcpy(out, pairs.data(), std::min<uint64_t>(capacity, pairs.size()) * sizeof(mvbt_pair));
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_node_width(mvbt_tree* t, uint32_t* nw) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!nw) return fail(MVBT_ERR_INVALID_ARG, "null out");
+  *nw = t->nw;
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_kind_of(mvbt_tree* t, uint32_t* kind) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!kind) return fail(MVBT_ERR_INVALID_ARG, "null out");
+  *kind = t->versioned ? MVBT_VERSIONED : MVBT_PLAIN;
+  return MVBT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// Batch kernels (reference btree_kernels.hpp:308-665). Each tile takes a chunk
+// of W consecutive operations (lane j holds operation j, so inputs and
+// outputs move coalesced), then runs them one at a time cooperatively
+// (ballot -> lowest lane -> shuffle -> cooperative op), the reference's
+// warp-cooperative work sharing (btree_kernels.hpp:507-517). Chunks are taken
+// grid-stride so a launch can be sized to residency instead of to n.
+#pragma once
+#include "mvbt_tree.hpp"
+
+namespace mvbt {
+
+template <int W>
+struct chunking {
+  uint64_t first, stride;
+  MVBT_DEV chunking() {
+    const uint64_t tiles_per_block = blockDim.x / W;
+    first = ((uint64_t)blockIdx.x * tiles_per_block + threadIdx.x / W) * W;
+    stride = (uint64_t)gridDim.x * tiles_per_block * W;
+  }
+};
+
+// Creates the root (internal, pivot 0 -> first leaf) and the first, empty leaf.
+// Reference initialize_kernel / allocate_root_node (btree_kernels.hpp:525-534,
+// gpu_versioned_blink_tree.hpp:1454-1475); here the root is never a leaf.
+template <int NW, bool VER>
+__global__ void init_kernel(tree_view<NW, VER> tv) {
+  using node_t = node<NW, VER>;
+  tile<NW> t;
+  if (blockIdx.x != 0 || threadIdx.x >= (unsigned)NW) return;
+  const uint32_t root = 0, leaf = 1;
+  uint32_t r = t.rank();
+  uint64_t rp, lp;
+  if (r == node_t::kMetaLane) {
+    rp = pack(kEmpty, kNoSibling);
+    lp = pack(kEmpty, kNoSibling | kLeafBit);
+  } else if (VER && r == node_t::kVerLane) {
+    rp = pack(0u, kNone);
+    lp = pack(tv.st->clock, kNone);
+  } else {
+    rp = (r == 0) ? pack(0u, leaf) : ~0ull;
+    lp = ~0ull;
+  }
+  tv.at(root)[r] = rp;
+  tv.at(leaf)[r] = lp;
+}
+
+template <int NW, bool VER, bool CONC>
+__global__ void __launch_bounds__(256) find_kernel(tree_view<NW, VER> tv, const uint32_t* keys,
+                                                   uint32_t* values, uint64_t n, uint32_t ts) {
+  tile<NW> t;
+  chunking<NW> c;
+  for (uint64_t base = c.first; base < n; base += c.stride) {
+    uint64_t i = base + t.rank();
+    bool active = i < n;
+    uint32_t key = active ? keys[i] : 0u;
+    uint32_t result = kEmpty;
+    uint64_t q = t.ballot(active);
+    while (q) {
+      uint32_t j = (uint32_t)lowest_bit(q);
+      q &= q - 1;
+      uint32_t r = tv.template find<CONC>(t.shfl(key, j), ts, t);
+      if (t.rank() == j) result = r;
+    }
+    if (active) values[i] = result;
+  }
+}
+
+template <int NW, bool VER, bool CONC>
+__global__ void __launch_bounds__(256)
+    range_kernel(tree_view<NW, VER> tv, const uint32_t* lower, const uint32_t* upper,
+                 uint64_t* out, uint32_t* counts, uint32_t L, uint64_t n, uint32_t ts) {
+  tile<NW> t;
+  chunking<NW> c;
+  for (uint64_t base = c.first; base < n; base += c.stride) {
+    uint64_t i = base + t.rank();
+    bool active = i < n;
+    uint32_t lo = active ? lower[i] : 0u;
+    uint32_t hi = active ? upper[i] : 0u;
+    uint32_t result = 0;
+    uint64_t q = t.ballot(active);
+    while (q) {
+      uint32_t j = (uint32_t)lowest_bit(q);
+      q &= q - 1;
+      uint64_t* slot = out ? out + (base + j) * (uint64_t)L : nullptr;
+      uint32_t cnt = tv.template range<CONC>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t);
+      if (t.rank() == j) result = cnt;
+    }
+    if (active && counts) counts[i] = result;
+    if (t.any(active && result > L) && t.leader()) atomic_or32(&tv.st->error_flags, kErrRangeOverflow);
+  }
+}
+
+template <int NW, bool VER, int OP>
+__global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const uint32_t* keys,
+                                                    const uint32_t* values, uint64_t n) {
+  tile<NW> t;
+  chunking<NW> c;
+  for (uint64_t base = c.first; base < n; base += c.stride) {
+    uint64_t i = base + t.rank();
+    bool active = i < n;
+    uint32_t key = active ? keys[i] : 0u;
+    uint32_t val = (active && OP == 0) ? values[i] : 0u;
+    uint64_t q = t.ballot(active);
+    while (q) {
+      uint32_t j = (uint32_t)lowest_bit(q);
+      q &= q - 1;
+      tv.template write<OP>(t.shfl(key, j), OP == 0 ? t.shfl(val, j) : 0u, t);
+    }
+  }
+}
+
+// take_snapshot_kernel (btree_kernels.hpp:473-478): one lane reads the retire
+// ring position, then advances the clock; both go to `rec`.
+__global__ void snapshot_kernel(tree_state* st, snapshot_record* rec) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  fence_release();
+  unsigned long long pos =
+      __hip_atomic_load(&st->retire_tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t s = __hip_atomic_fetch_add(&st->clock, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&rec->ring_pos, pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&rec->id, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Reclamation, phase 1 (one lane): frees retire-ring entries [head, end) by
+// moving them onto the free stack. Records the live count before freeing as a
+// peak candidate.
+__global__ void reclaim_prep_kernel(tree_state* st, unsigned long long end_pos) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long head = st->retire_head, tail = st->retire_tail;
+  unsigned long long end = end_pos < tail ? end_pos : tail;
+  if (end < head) end = head;
+  unsigned long long count = end - head;
+  // Positions at or beyond head + cap overflowed and were never stored.
+  if (count > st->retire_cap) count = st->retire_cap;
+  long long top = st->free_top;
+  unsigned long long base = top > 0 ? (unsigned long long)top : 0ull;
+  unsigned long long used = st->bump < st->capacity ? st->bump : st->capacity;
+  unsigned long long live = used - base;
+  if (live > st->peak_live) st->peak_live = live;
+  st->free_top = (long long)(base + count);
+  st->scratch[0] = head;  // first ring position to copy
+  st->scratch[1] = count;
+  st->scratch[2] = base;
+  st->retire_head = end;
+  st->reclaimed += count;
+}
+
+// Reclamation, phase 2: grid-stride copy ring -> free stack.
+__global__ void reclaim_copy_kernel(tree_state* st, const uint32_t* ring, uint32_t* free_stack) {
+  const unsigned long long start = st->scratch[0], count = st->scratch[1], base = st->scratch[2];
+  const unsigned long long mask = st->retire_cap - 1;
+  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < count;
+       i += (unsigned long long)gridDim.x * blockDim.x) {
+    free_stack[base + i] = ring[(start + i) & mask];
+  }
+}
+
+}  // namespace mvbt

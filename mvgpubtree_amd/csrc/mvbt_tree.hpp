@@ -1,0 +1,492 @@
+// Device-side multiversion B-link tree over uint32 keys and values.
+//
+// Reference behaviour restated here (all paths under the reference's include/):
+//   * cooperative_find(key, ts)            gpu_versioned_blink_tree.hpp:439-474,
+//                                          plain gpu_blink_tree.hpp:278-303
+//   * concurrent_cooperative_range_query   gpu_versioned_blink_tree.hpp:505-559
+//   * cooperative_insert_versioned_in_place gpu_versioned_blink_tree.hpp:825-1086,
+//     plain cooperative_insert_blink       gpu_blink_tree.hpp:523-733
+//   * cooperative_erase                    gpu_versioned_blink_tree.hpp:356-436
+//   * take_snapshot(tile)                  gpu_versioned_blink_tree.hpp:333-344
+//   * traverse_version_list / side links   gpu_versioned_blink_tree.hpp:1480-1603
+//
+// Design (differs from the reference on purpose):
+//   * Only leaves carry versions. Internal nodes are routing data: every reader,
+//     whatever its snapshot, routes through the latest internal nodes and then
+//     walks the leaf's version chain to the newest version stamped <= snapshot.
+//     Because keys only ever move right (splits, never merges) and a leaf made
+//     by a split inherits the version chain of the leaf it came from, the
+//     version found is the leaf that held the key at the snapshot. A split
+//     therefore copies at most the split leaf, never its parent.
+//   * The root is internal from creation and never moves, so no node ever
+//     changes between leaf and internal: a torn read of an internal node mixes
+//     {pivot, child} pairs of two valid versions and can only route too far
+//     left, which the B-link right move repairs.
+//   * Writers take locks with try_lock only and restart on failure; no tile
+//     ever spins on a lock inside a loop of its own, because the 64/W tiles of
+//     a wave execute in lockstep and a spinning tile would starve its holder.
+//   * A leaf modified in a new epoch is first copied (the copy keeps the old
+//     content and stamp), then its version word {cur, copy} is published, and
+//     only then are its data lanes rewritten, each step behind a release fence.
+//     A concurrent snapshot reader that re-reads the version word after reading
+//     the leaf therefore detects any overlapping rewrite.
+#pragma once
+#include "mvbt_node.hpp"
+
+namespace mvbt {
+
+// Device-resident state of one tree (one per handle).
+struct tree_state {
+  uint32_t root;          // index of the root node (fixed after creation)
+  uint32_t clock;         // write stamp; take_snapshot returns clock++
+  uint32_t error_flags;   // kErr* bits
+  uint32_t pad0;
+  unsigned long long bump;         // next never-used node index
+  unsigned long long capacity;     // pool capacity in nodes
+  long long free_top;              // recycled indices available in free_stack
+  unsigned long long retire_tail;  // total retirements appended to the ring
+  unsigned long long retire_head;  // first retirement not yet reclaimed
+  unsigned long long retire_cap;   // ring capacity (power of two)
+  unsigned long long reclaimed;    // total nodes returned to the free stack
+  unsigned long long peak_live;    // max live nodes observed at reclaim/stats
+  unsigned long long scratch[4];   // host<->device hand-off words
+};
+
+// Result of a snapshot taken on the device: the id and the retire-ring
+// position at the epoch boundary (used by epoch reclamation).
+struct snapshot_record {
+  uint32_t id;
+  uint32_t pad;
+  unsigned long long ring_pos;
+};
+
+template <int NW, bool VER>
+struct tree_view {
+  using node_t = node<NW, VER>;
+  using tile_t = tile<NW>;
+  static constexpr int kWidth = NW;
+  static constexpr bool kVersioned = VER;
+
+  uint64_t* nodes;         // pool: node i occupies nodes[i*NW .. i*NW+NW)
+  tree_state* st;
+  uint32_t* free_stack;    // recycled node indices, [0, free_top)
+  uint32_t* retire_ring;   // retired node indices, ring of retire_cap
+  uint32_t root_;
+  uint32_t cap_;           // pool capacity: every followed index must be below it
+
+  MVBT_DEV uint64_t* at(uint32_t idx) const { return nodes + (size_t)idx * NW; }
+  MVBT_DEV uint32_t* lock_word(uint32_t idx) const {
+    return reinterpret_cast<uint32_t*>(at(idx) + node_t::kMetaLane) + 1;
+  }
+  MVBT_DEV void flag(uint32_t bits, const tile_t& t) const {
+    if (t.leader()) atomic_or32(&st->error_flags, bits);
+  }
+
+  template <bool CONC>
+  MVBT_DEV void load(node_t& n, uint32_t idx, const tile_t& t) const {
+    if constexpr (CONC) n.load(at(idx), t);
+    else n.load_weak(at(idx), t);
+  }
+
+  // ---- allocation (reference device_bump_allocator.hpp:45-54 and
+  // slab_alloc.hpp:131-205 roles). Pops a recycled index, else bumps the
+  // frontier. Returns kNone when the pool is exhausted.
+  MVBT_DEV uint32_t alloc(const tile_t& t) const {
+    uint32_t idx = kNone;
+    if (t.leader()) {
+      long long top = atomic_add_s64(&st->free_top, -1);
+      if (top > 0) {
+        idx = ld32_coherent(free_stack + (top - 1));
+      } else {
+        unsigned long long b = atomic_add64(&st->bump, 1ull);
+        if (b < st->capacity) idx = (uint32_t)b;
+        else atomic_or32(&st->error_flags, kErrPoolExhausted);
+      }
+    }
+    return t.shfl(idx, 0);
+  }
+  // Epoch retirement of a version copy (reference DEBR_device::retire,
+  // memory_reclaimer.hpp:146-228). The copy becomes garbage once no snapshot
+  // older than the epoch that created it is live.
+  MVBT_DEV void retire(uint32_t idx, const tile_t& t) const {
+    if (t.leader()) {
+      unsigned long long pos = atomic_add64(&st->retire_tail, 1ull);
+      unsigned long long head = __hip_atomic_load(&st->retire_head, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+      if (pos - head < st->retire_cap) {
+        st32_coherent(retire_ring + (pos & (st->retire_cap - 1)), idx);
+      } else {
+        // The slot still holds an unreclaimed entry: leave it; this copy leaks.
+        atomic_or32(&st->error_flags, kErrRetireOverflow);
+      }
+    }
+  }
+
+  // ---- locks (reference versioned_node.hpp:528-568; here try-lock only) ----
+  MVBT_DEV bool try_lock(uint32_t idx, const tile_t& t) const {
+    uint32_t old = 0;
+    if (t.leader()) old = atomic_or32(lock_word(idx), kLockBit);
+    return (t.shfl(old, 0) & kLockBit) == 0;
+  }
+  MVBT_DEV void unlock(uint32_t idx, const tile_t& t) const {
+    fence_release();
+    if (t.leader()) atomic_and32(lock_word(idx), ~kLockBit);
+  }
+
+  // ---- snapshots ------------------------------------------------------------
+  MVBT_DEV uint32_t current_stamp() const { return ld32_coherent(&st->clock); }
+  // Takes a snapshot: returns the id s; writes after this call are stamped > s.
+  MVBT_DEV uint32_t take_snapshot(const tile_t& t) const {
+    uint32_t s = 0;
+    if (t.leader()) {
+      fence_release();
+      s = __hip_atomic_fetch_add(&st->clock, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return t.shfl(s, 0);
+  }
+
+  // Re-reads a leaf's version word (leader load, broadcast).
+  MVBT_DEV uint64_t reread_version(uint32_t idx, const tile_t& t) const {
+    uint64_t v = 0;
+    if (t.leader()) v = ld_coherent(at(idx) + node_t::kVerLane);
+    return t.shfl64(v, 0);
+  }
+
+  // Moves `n` (the leaf stored at `idx`) to its newest version stamped <= ts.
+  // Returns false if the key range did not exist at ts (version gap).
+  template <bool CONC>
+  MVBT_DEV bool to_version(node_t& n, uint32_t idx, uint32_t ts, const tile_t& t) const {
+    if constexpr (!VER) {
+      return true;
+    } else {
+      if (ts == kLatest) return true;
+      uint64_t ver = n.version(t);
+      if (node_t::stamp(ver) <= ts) {
+        if constexpr (!CONC) return true;
+        // Validate: the leaf's version word must not have moved while we read it.
+        for (uint32_t step = 0;; ++step) {
+          if (step >= kMaxReadSteps) {
+            flag(kErrCorrupt, t);
+            return false;
+          }
+          uint64_t ver2 = reread_version(idx, t);
+          if (ver2 == ver) return true;
+          ver = ver2;
+          if (node_t::stamp(ver) > ts) break;
+          // Stamp still visible but the word moved: read the leaf again.
+          load<true>(n, idx, t);
+          ver = n.version(t);
+          if (node_t::stamp(ver) > ts) break;
+        }
+      }
+      for (uint32_t step = 0; node_t::stamp(ver) > ts; ++step) {
+        uint32_t o = node_t::older(ver);
+        if (o == kNone) {
+          flag(kErrVersionGap, t);
+          return false;
+        }
+        if (o >= cap_ || step >= kMaxReadSteps) {
+          flag(kErrCorrupt, t);
+          return false;
+        }
+        load<CONC>(n, o, t);  // version copies are immutable once published
+        ver = n.version(t);
+      }
+      return true;
+    }
+  }
+
+  // Descends (latest routing, B-link right moves) to the leaf covering `key`.
+  // Returns kNone (and flags kErrCorrupt) if the walk leaves the pool.
+  template <bool CONC>
+  MVBT_DEV uint32_t descend(uint32_t key, node_t& n, const tile_t& t) const {
+    uint32_t idx = root_;
+    for (uint32_t step = 0; step < kMaxReadSteps; ++step) {
+      load<CONC>(n, idx, t);
+      uint64_t m = n.meta(t);
+      uint32_t next;
+      if (key >= node_t::high_key(m)) {
+        next = node_t::sibling(m);
+      } else if (node_t::is_leaf(m)) {
+        return idx;
+      } else {
+        next = n.child_for(key, t);
+      }
+      if (next >= cap_) break;
+      idx = next;
+    }
+    flag(kErrCorrupt, t);
+    return kNone;
+  }
+
+  // ---- find -----------------------------------------------------------------
+  template <bool CONC>
+  MVBT_DEV uint32_t find(uint32_t key, uint32_t ts, const tile_t& t) const {
+    if (!valid_key(key)) return kEmpty;
+    node_t n;
+    uint32_t idx = descend<CONC>(key, n, t);
+    if (idx == kNone || !to_version<CONC>(n, idx, ts, t)) return kEmpty;
+    return n.lookup(key, t);
+  }
+
+  // ---- range query [lo, hi) ------------------------------------------------
+  // Writes min(count, cap) pairs to out[0..) and returns the true count.
+  template <bool CONC>
+  MVBT_DEV uint32_t range(uint32_t lo, uint32_t hi, uint32_t ts, uint64_t* out, uint32_t cap,
+                          const tile_t& t) const {
+    if (lo == 0) lo = 1;
+    if (hi <= lo) return 0;
+    node_t n;
+    uint32_t idx = descend<CONC>(lo, n, t);
+    if (idx == kNone || !to_version<CONC>(n, idx, ts, t)) return 0;
+    uint32_t count = 0;
+    for (uint32_t step = 0;; ++step) {
+      count += n.collect(lo, hi, out, count, cap, t);
+      uint64_t m = n.meta(t);
+      uint32_t sib = node_t::sibling(m);
+      if (hi <= node_t::high_key(m) || sib == kNoSibling) break;
+      if (sib >= cap_ || step >= kMaxReadSteps) {
+        flag(kErrCorrupt, t);
+        break;
+      }
+      idx = sib;
+      load<CONC>(n, idx, t);
+      if (!to_version<CONC>(n, idx, ts, t)) break;
+    }
+    return count;
+  }
+
+  // ---- writers ----------------------------------------------------------------
+  enum step : uint32_t { kAtRoot = 0, kAtParent = 1, kOutOfMemory = 2 };
+
+  // Splits the full root in place: its pairs move to two new children and the
+  // root keeps {pivot0 -> left, pivot -> right} (reference split_as_root,
+  // versioned_node.hpp:185-212, with the root never a leaf here).
+  MVBT_DEV uint32_t split_root(const tile_t& t) const {
+    uint32_t r = root_;
+    if (!try_lock(r, t)) return kAtRoot;
+    node_t n;
+    n.load(at(r), t);
+    if (!n.is_full(t)) {
+      unlock(r, t);
+      return kAtRoot;
+    }
+    uint32_t left = alloc(t);
+    uint32_t right = (left == kNone) ? kNone : alloc(t);
+    if (right == kNone) {
+      unlock(r, t);
+      return kOutOfMemory;
+    }
+    uint64_t m = n.meta(t);
+    uint32_t pivot = n.split_pivot(t);
+    const uint64_t internal_ver = pack(0u, kNone);
+    node_t rn = n.upper_half(m, internal_ver, t);
+    node_t ln = n;
+    ln.keep_lower_half(pivot, right, t);
+    ln.set_lock_in_registers(false, t);
+    if (VER) ln.set_lane(node_t::kVerLane, internal_ver, t);
+    ln.store(at(left), t);
+    rn.store(at(right), t);
+    fence_release();
+    uint32_t k0 = lo32(n.lane_pair(0, t));
+    uint32_t rk = t.rank();
+    if (node_t::data_lane(rk)) {
+      n.p = (rk == 0) ? pack(k0, left) : (rk == 1) ? pack(pivot, right) : node_t::kEmptyPair;
+    }
+    n.store_body(at(r), t);
+    unlock(r, t);
+    return kAtRoot;
+  }
+
+  // Splits full internal node `idx` whose parent is `parent` (reference
+  // split, versioned_node.hpp:149-178; parent locked and re-checked as in
+  // gpu_versioned_blink_tree.hpp:934-1009, but with try_lock and restart).
+  MVBT_DEV uint32_t split_internal(uint32_t idx, uint32_t parent, uint32_t key,
+                                   const tile_t& t) const {
+    if (!try_lock(idx, t)) return kAtParent;
+    node_t n;
+    n.load(at(idx), t);
+    uint64_t m = n.meta(t);
+    if (!n.is_full(t) || key >= node_t::high_key(m)) {
+      unlock(idx, t);
+      return kAtParent;
+    }
+    if (!try_lock(parent, t)) {
+      unlock(idx, t);
+      return kAtParent;
+    }
+    node_t pn;
+    pn.load(at(parent), t);
+    if (pn.is_full(t) || !pn.has_child(idx, t)) {
+      unlock(parent, t);
+      unlock(idx, t);
+      return kAtRoot;
+    }
+    uint32_t y = alloc(t);
+    if (y == kNone) {
+      unlock(parent, t);
+      unlock(idx, t);
+      return kOutOfMemory;
+    }
+    uint32_t pivot = n.split_pivot(t);
+    node_t yn = n.upper_half(m, pack(0u, kNone), t);
+    yn.store(at(y), t);
+    fence_release();
+    n.keep_lower_half(pivot, y, t);
+    n.store(at(idx), t);  // metadata lane keeps the lock bit
+    fence_release();
+    pn.insert(pivot, y, t);
+    pn.store_body(at(parent), t);
+    unlock(parent, t);
+    unlock(idx, t);
+    return kAtParent;
+  }
+
+  // Publishes a new epoch for locked leaf `idx` if its stamp is older than
+  // `cur`: copies the current content out, publishes {cur, copy} in the
+  // version word, fenced on both sides. Returns false when out of memory.
+  MVBT_DEV bool open_epoch(node_t& n, uint32_t idx, uint32_t cur, const tile_t& t) const {
+    if constexpr (VER) {
+      uint64_t ver = n.version(t);
+      if (node_t::stamp(ver) != cur) {
+        uint32_t c = alloc(t);
+        if (c == kNone) return false;
+        n.store_unlocked_copy(at(c), t);
+        retire(c, t);
+        fence_release();
+        uint64_t nv = pack(cur, c);
+        if (t.leader()) st_coherent(at(idx) + node_t::kVerLane, nv);
+        n.set_lane(node_t::kVerLane, nv, t);
+        fence_release();
+      }
+    }
+    return true;
+  }
+
+  // Splits locked full leaf `idx` (loaded in `n`) under locked-and-checked
+  // parent; always unlocks `idx`.
+  MVBT_DEV uint32_t split_leaf(node_t& n, uint32_t idx, uint32_t parent, uint32_t cur,
+                               const tile_t& t) const {
+    if (parent == kNone) {
+      unlock(idx, t);
+      return kAtRoot;
+    }
+    if (!try_lock(parent, t)) {
+      unlock(idx, t);
+      return kAtParent;
+    }
+    node_t pn;
+    pn.load(at(parent), t);
+    if (pn.is_full(t) || !pn.has_child(idx, t)) {
+      unlock(parent, t);
+      unlock(idx, t);
+      return kAtRoot;
+    }
+    uint32_t y = alloc(t);
+    if (y == kNone || !open_epoch(n, idx, cur, t)) {
+      unlock(parent, t);
+      unlock(idx, t);
+      return kOutOfMemory;  // an allocated y is simply never linked; counted as leaked
+    }
+    uint64_t m = n.meta(t);
+    uint32_t pivot = n.split_pivot(t);
+    uint64_t ver = VER ? n.version(t) : 0ull;
+    node_t yn = n.upper_half(m, ver, t);
+    yn.store(at(y), t);
+    fence_release();
+    n.keep_lower_half(pivot, y, t);
+    if constexpr (VER) {
+      if (t.rank() != node_t::kVerLane) st_coherent(at(idx) + t.rank(), n.p);
+    } else {
+      n.store(at(idx), t);
+    }
+    fence_release();
+    pn.insert(pivot, y, t);
+    pn.store_body(at(parent), t);
+    unlock(parent, t);
+    unlock(idx, t);
+    return kAtParent;
+  }
+
+  // Shared writer loop. OP = 0 insert/update, 1 erase. Returns true if the
+  // tree changed (insert: always when it succeeds; erase: key was present).
+  template <int OP>
+  MVBT_DEV bool write(uint32_t key, uint32_t value, const tile_t& t) const {
+    if (!valid_key(key)) {
+      flag(kErrInvalidKey, t);
+      return false;
+    }
+    const uint32_t cur = VER ? current_stamp() : 0u;
+    uint32_t idx = root_, parent = kNone;
+    node_t n;
+    for (uint32_t step = 0;; ++step) {
+      if (idx >= cap_ || step >= kMaxWriteSteps) {  // no lock is held here
+        flag(kErrCorrupt, t);
+        return false;
+      }
+      n.load(at(idx), t);
+      uint64_t m = n.meta(t);
+      if (key >= node_t::high_key(m)) {
+        idx = node_t::sibling(m);
+        parent = kNone;
+        continue;
+      }
+      if (!node_t::is_leaf(m)) {
+        if (OP == 0 && n.is_full(t)) {  // proactive split on the way down
+          uint32_t s = (idx == root_) ? split_root(t)
+                       : (parent == kNone) ? (uint32_t)kAtRoot
+                                           : split_internal(idx, parent, key, t);
+          if (s == kOutOfMemory) return false;
+          if (s == kAtParent && parent != kNone) idx = parent;
+          else idx = root_;
+          parent = kNone;
+          continue;
+        }
+        parent = idx;
+        idx = n.child_for(key, t);
+        continue;
+      }
+      if (!try_lock(idx, t)) {
+        __builtin_amdgcn_s_sleep(2);
+        continue;  // re-read the same leaf
+      }
+      n.load(at(idx), t);
+      m = n.meta(t);
+      if (key >= node_t::high_key(m)) {
+        unlock(idx, t);
+        idx = node_t::sibling(m);
+        parent = kNone;
+        continue;
+      }
+      if (OP == 1) {
+        if (!n.contains(key, t)) {
+          unlock(idx, t);
+          return false;
+        }
+      } else if (n.is_full(t) && !n.contains(key, t)) {
+        uint32_t s = split_leaf(n, idx, parent, cur, t);
+        if (s == kOutOfMemory) return false;
+        idx = (s == kAtParent && parent != kNone) ? parent : root_;
+        parent = kNone;
+        continue;
+      }
+      if (!open_epoch(n, idx, cur, t)) {
+        unlock(idx, t);
+        return false;
+      }
+      if (OP == 0) n.insert(key, value, t);
+      else n.erase(key, t);
+      uint32_t rk = t.rank();
+      if (node_t::data_lane(rk)) st_coherent(at(idx) + rk, n.p);
+      unlock(idx, t);
+      return true;
+    }
+  }
+
+  MVBT_DEV bool insert(uint32_t key, uint32_t value, const tile_t& t) const {
+    return write<0>(key, value, t);
+  }
+  MVBT_DEV bool erase(uint32_t key, const tile_t& t) const { return write<1>(key, 0u, t); }
+};
+
+}  // namespace mvbt

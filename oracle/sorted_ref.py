@@ -1,0 +1,101 @@
+"""ORACLE - test infrastructure only.
+
+Batch-state restatement of the reference's validation semantics
+(owensgroup/MVGpuBTree include/validation.hpp:75-104 for finds,
+:184-264 for ranges packed at i*L) for full-size workloads: the latest key set
+is kept as sorted numpy arrays; every read of a batch is answered against the
+state before the batch (the batch's pre-batch snapshot), then the batch's
+writes are applied. Equivalent to ``MVMap`` reading at the snapshot taken
+before each batch (cross-checked in tests/test_oracle.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+EMPTY = np.uint32(0xFFFFFFFF)
+KEY_MAX = 0xFFFFFFFF
+
+
+def range_bounds(lower: np.ndarray, span: int) -> np.ndarray:
+    """Upper bounds lower + span, saturated at 0xFFFFFFFF (exclusive bound)."""
+    hi = lower.astype(np.uint64) + np.uint64(span)
+    return np.minimum(hi, np.uint64(KEY_MAX)).astype(np.uint32)
+
+
+class SortedState:
+    def __init__(self, keys=None, values=None):
+        if keys is None:
+            self.keys = np.empty(0, dtype=np.uint32)
+            self.values = np.empty(0, dtype=np.uint32)
+        else:
+            k = np.asarray(keys, dtype=np.uint32)
+            v = np.asarray(values, dtype=np.uint32)
+            order = np.argsort(k, kind="stable")
+            k, v = k[order], v[order]
+            # last write wins for duplicate keys
+            last = np.ones(k.size, dtype=bool)
+            if k.size > 1:
+                last[:-1] = k[1:] != k[:-1]
+            self.keys, self.values = k[last].copy(), v[last].copy()
+
+    def __len__(self):
+        return int(self.keys.size)
+
+    def _locate(self, q: np.ndarray):
+        q = np.asarray(q, dtype=np.uint32)
+        idx = np.searchsorted(self.keys, q, side="left")
+        safe = np.minimum(idx, max(self.keys.size - 1, 0))
+        hit = (idx < self.keys.size) & (self.keys[safe] == q) if self.keys.size else np.zeros(q.size, bool)
+        return idx, hit
+
+    def find(self, q) -> np.ndarray:
+        q = np.asarray(q, dtype=np.uint32)
+        idx, hit = self._locate(q)
+        out = np.full(q.size, EMPTY, dtype=np.uint32)
+        if self.keys.size:
+            out[hit] = self.values[idx[hit]]
+        out[(q == 0) | (q == EMPTY)] = EMPTY
+        return out
+
+    def range_index(self, lower, upper):
+        lo = np.maximum(np.asarray(lower, dtype=np.uint32), np.uint32(1))
+        hi = np.asarray(upper, dtype=np.uint32)
+        a = np.searchsorted(self.keys, lo, side="left")
+        b = np.searchsorted(self.keys, hi, side="left")
+        return a, np.maximum(b - a, 0).astype(np.int64)
+
+    def range_counts(self, lower, upper) -> np.ndarray:
+        return self.range_index(lower, upper)[1].astype(np.uint32)
+
+    def range_query(self, lower, upper, L: int):
+        """Returns (pairs u64[n*L], counts u32[n]); unused slots are all-ones."""
+        a, c = self.range_index(lower, upper)
+        n = a.size
+        j = np.arange(L, dtype=np.int64)[None, :]
+        take = j < np.minimum(c, L)[:, None]
+        src = np.minimum(a[:, None] + j, max(self.keys.size - 1, 0))
+        packed = self.keys.astype(np.uint64) | (self.values.astype(np.uint64) << np.uint64(32))
+        out = np.full((n, L), np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+        if self.keys.size:
+            out[take] = packed[src[take]]
+        return out.reshape(-1), c.astype(np.uint32)
+
+    def insert(self, keys, values):
+        k = np.asarray(keys, dtype=np.uint32)
+        v = np.asarray(values, dtype=np.uint32)
+        ok = (k != 0) & (k != EMPTY)
+        k, v = k[ok], v[ok]
+        batch = SortedState(k, v)  # dedupe inside the batch, last wins
+        k, v = batch.keys, batch.values
+        idx, hit = self._locate(k)
+        self.values[idx[hit]] = v[hit]
+        new = ~hit
+        self.keys = np.insert(self.keys, idx[new], k[new])
+        self.values = np.insert(self.values, idx[new], v[new])
+
+    def erase(self, keys) -> int:
+        k = np.unique(np.asarray(keys, dtype=np.uint32))
+        idx, hit = self._locate(k)
+        self.keys = np.delete(self.keys, idx[hit])
+        self.values = np.delete(self.values, idx[hit])
+        return int(hit.sum())
