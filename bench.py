@@ -1,0 +1,258 @@
+"""Benchmark: M ops/s of the multiversion B-link tree on one MI355X.
+
+Default workload (the metric's configuration): versioned tree, uint32 keys and
+values, bulk insert of 40M unique uniform keys, then batches of 1M ops each:
+500K inserts of new keys and 500K range queries [lo, lo + span) holding ~32
+keys (result slots L = 128), every range reading the snapshot taken before its
+batch. One step = one batch: take_snapshot, range_query at it, insert,
+release the snapshot, reclaim. Inputs are resident in HBM before timing.
+
+Prints one JSON line with the contract fields plus:
+  roofline      dominant kernel's algorithmic bytes / average launch time
+                against 8 TB/s (HIP events on the launch stream);
+  cpu_baseline  the oracle (single-thread std::map restatement of the
+                reference's serial validation path) on one batch of the same
+                workload on this host;
+  breakdown     per-operation rates, plus 1M finds (50% hits) on the same tree.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per operation at 40M keys: reference layout, 128-B nodes,
+# h = 8 levels (insert adds leaf write + version copy; range-32 adds 5 leaves
+# and 32 result pairs).
+H_REF = 8
+BYTES = {
+    "find": H_REF * 128 + 8,
+    "insert": H_REF * 128 + 2 * 128 + 8,
+    "range32": H_REF * 128 + 5 * 128 + 32 * 8 + 12,
+    "erase": H_REF * 128 + 2 * 128 + 4,
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def dev_tensor(a, dev):
+    import torch
+    a = np.ascontiguousarray(a)
+    return torch.from_numpy(a.view(np.int32 if a.dtype.itemsize == 4 else np.int64)).to(dev)
+
+
+def cpu_baseline_config2(w, batch):
+    """Times the oracle (std::map version histories) on one batch of the
+    workload: range queries at the pre-batch snapshot, then the inserts."""
+    import oracle
+    m = oracle.MVMap()
+    ks = np.sort(w.initial_keys)
+    m.build_sorted(ks, (ks ^ np.uint32(0x9E3779B9)).astype(np.uint32))
+    t0 = time.perf_counter()
+    s = m.take_snapshot()
+    m.range_query(batch.range_lo, batch.range_hi, w.range_capacity, s)
+    m.insert(batch.insert_keys, batch.insert_values)
+    dt = time.perf_counter() - t0
+    ops = batch.num_ops
+    m.close()
+    return {"value": round(ops / dt / 1e6, 3), "unit": "M ops/s", "cores": 1, "kind": "port",
+            "sample": f"1 batch ({ops} ops: {batch.range_lo.size} range-32 + "
+                      f"{batch.insert_keys.size} inserts) on the {w.initial_keys.size}-key map, "
+                      f"{dt:.2f} s on one host core"}
+
+
+def run_config2(args):
+    import torch
+
+    import mvgpubtree_amd as mv
+    from mvgpubtree_amd import workloads as wl
+
+    dev = torch.device("cuda:0")
+    n_initial = args.keys
+    t0 = time.perf_counter()
+    w = wl.config2(seed=2, n_initial=n_initial, num_batches=args.steps, ops_per_batch=args.ops)
+    # Warmup runs the same step on a separate 1M-key tree, so the timed batches
+    # start from exactly the configured build.
+    ww = wl.config2(seed=99, n_initial=min(n_initial, 1_000_000), num_batches=max(args.warmup, 1),
+                    ops_per_batch=args.ops)
+    log(f"workload generated in {time.perf_counter() - t0:.1f}s (span {w.span})")
+    L = w.range_capacity
+    batches = w.batches
+    dbat = [(dev_tensor(b.insert_keys, dev), dev_tensor(b.insert_values, dev),
+             dev_tensor(b.range_lo, dev), dev_tensor(b.range_hi, dev)) for b in batches]
+    n_rq = batches[0].range_lo.size
+    out_pairs = torch.empty(n_rq * L, dtype=torch.int64, device=dev)
+    out_counts = torch.empty(n_rq, dtype=torch.int32, device=dev)
+
+    def make_tree(n):
+        pool = int((n + args.steps * args.ops) / 4) + (1 << 20)
+        return mv.GpuVersionedBTree(pool_nodes=pool, node_width=args.node_width,
+                                    retire_capacity=1 << 23)
+
+    scratch = make_tree(ww.initial_keys.size)
+    scratch.insert(dev_tensor(ww.initial_keys, dev), dev_tensor(ww.initial_values, dev))
+    wbat = [(dev_tensor(b.insert_keys, dev), dev_tensor(b.insert_values, dev),
+             dev_tensor(b.range_lo, dev), dev_tensor(b.range_hi, dev)) for b in ww.batches]
+
+    tree = make_tree(n_initial)
+    dk, dv = dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tree.insert(dk, dv)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    log(f"built {n_initial} keys in {build_s:.3f}s ({n_initial / build_s / 1e6:.1f} M inserts/s)")
+    del dk, dv
+
+    stream = torch.cuda.current_stream()
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+    kt = {"range32": [], "insert": []}
+
+    def step(tree, batch, timed):
+        ik, iv, lo, hi = batch
+        s = tree.take_snapshot(stream)
+        e0, e1, e2 = ev(), ev(), ev()
+        e0.record(stream)
+        tree.range_query(lo, hi, L, result=out_pairs, counts=out_counts, timestamp=s, stream=stream)
+        e1.record(stream)
+        tree.insert(ik, iv, stream=stream)
+        e2.record(stream)
+        tree.release_snapshot(s)
+        tree.reclaim(stream)
+        if timed:
+            kt["range32"].append((e0, e1))
+            kt["insert"].append((e1, e2))
+
+    for b in range(args.warmup):
+        step(scratch, wbat[b], False)
+    torch.cuda.synchronize()
+    scratch.close()
+    t0 = time.perf_counter()
+    for b in range(args.steps):
+        step(tree, dbat[b], True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ops = sum(b.num_ops for b in batches)
+
+    kernel_ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in kt.items()}
+    n_ins = batches[0].insert_keys.size
+    rates = {"range32_Mops": n_rq / kernel_ms["range32"] / 1e3,
+             "insert_Mops": n_ins / kernel_ms["insert"] / 1e3}
+    flags = tree.error_flags()
+    if flags:
+        raise RuntimeError(f"device error flags 0x{flags:x}")
+
+    # 1M finds (50% hits) on the same tree at the latest snapshot.
+    rng = np.random.default_rng(77)
+    present = w.initial_keys[rng.integers(0, n_initial, size=args.ops // 2)]
+    absent = wl.absent_keys(rng, args.ops - args.ops // 2, np.sort(np.concatenate(
+        [w.initial_keys] + [b.insert_keys for b in batches])))
+    q = np.concatenate([present, absent])
+    rng.shuffle(q)
+    dq = dev_tensor(q, dev)
+    fout = torch.empty(q.size, dtype=torch.int32, device=dev)
+    sf = tree.take_snapshot(stream)
+    for _ in range(2):
+        tree.find(dq, fout, timestamp=sf, stream=stream)
+    fe = []
+    for _ in range(5):
+        a, b = ev(), ev()
+        a.record(stream)
+        tree.find(dq, fout, timestamp=sf, stream=stream)
+        b.record(stream)
+        fe.append((a, b))
+    torch.cuda.synchronize()
+    find_ms = float(np.mean([a.elapsed_time(b) for a, b in fe]))
+    hits = int((fout.cpu().numpy().view(np.uint32) != 0xFFFFFFFF).sum())
+    rates["find_Mops"] = q.size / find_ms / 1e3
+    tree.release_snapshot(sf)
+    mem = tree.memory_stats()
+
+    # Dominant kernel of the step and its roofline.
+    share = {"range32": kernel_ms["range32"], "insert": kernel_ms["insert"]}
+    dom = max(share, key=share.get)
+    per_launch_bytes = (n_rq if dom == "range32" else n_ins) * BYTES[dom]
+    achieved = per_launch_bytes / (kernel_ms[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"range_kernel<16,versioned> ({n_rq} range-32 queries/launch)"
+                if dom == "range32" else f"write_kernel<16,versioned,insert> ({n_ins} inserts/launch)",
+                "avg_launch_ms": round(kernel_ms[dom], 4)}
+    traffic = load_traffic(dom)
+    if traffic is not None:
+        roofline["traffic"] = traffic
+
+    value = ops / elapsed / 1e6
+    result = {
+        "metric": "M ops/s (find, insert, range-32) at 40M keys; p99 latency served vs plain",
+        "value": round(value, 2),
+        "unit": "M ops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "uint32",
+        "data": "synthetic (seeded uniform unique uint32 keys, value = key ^ 0x9E3779B9)",
+        "config": {"workload": "versioned tree, 40M-key build, batches of 1M ops: 50% inserts of "
+                               "new keys + 50% range queries (~32 keys, L=128) at the pre-batch "
+                               "snapshot; plain per-batch launch path",
+                   "initial_keys": n_initial, "ops_per_batch": args.ops, "node_width": args.node_width,
+                   "range_span": w.span, "range_capacity": L},
+        "roofline": roofline,
+        "breakdown": {k: round(v, 2) for k, v in rates.items()} | {
+            "build_Minserts_per_s": round(n_initial / build_s / 1e6, 2),
+            "find_hits": hits,
+            "find_roofline_frac": round(q.size * BYTES["find"] / (find_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "live_nodes": mem.live_nodes, "peak_live_nodes": mem.peak_live_nodes,
+            "reclaimed_nodes": mem.reclaimed_nodes},
+    }
+    tree.close()
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline_config2(w, batches[0])
+    return result
+
+
+def load_traffic(kernel: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, copy-kernel calibrated), when a summary has been recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--keys", type=int, default=40_000_000)
+    ap.add_argument("--ops", type=int, default=1_000_000)
+    ap.add_argument("--node-width", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the tree is a single-device structure here "
+              "(multi-GPU key-range sharding is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    result = run_config2(args)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -46,11 +46,18 @@ struct mvbt_tree {
   uint64_t* nodes = nullptr;
   tree_state* st = nullptr;
   uint32_t* free_stack = nullptr;
+  uint32_t* spill = nullptr;
   uint32_t* retire_ring = nullptr;
   snapshot_record* snap_rec = nullptr;  // pinned host slot
   std::mutex mu;
   std::map<uint32_t, uint32_t> live;                  // live snapshot id -> refcount
   std::vector<std::pair<uint32_t, uint64_t>> bounds;  // (snapshot id, ring pos at that epoch edge)
+  // Free-list maintenance (spill merge, reclamation) must not overlap a
+  // writer on another stream: writers wait for the last maintenance event,
+  // and maintenance runs only when every other stream's last write is done.
+  std::map<hipStream_t, hipEvent_t> write_ev;
+  hipEvent_t maint_ev = nullptr;
+  bool spill_maybe = false;
 
   template <int NW, bool VER>
   tree_view<NW, VER> view() const {
@@ -58,7 +65,9 @@ struct mvbt_tree {
     v.nodes = nodes;
     v.st = st;
     v.free_stack = free_stack;
+    v.spill = spill;
     v.retire_ring = retire_ring;
+    v.retire_cap = cfg.retire_capacity;
     v.root_ = 0;
     v.cap_ = (uint32_t)cfg.pool_nodes;
     return v;
@@ -163,6 +172,46 @@ const dispatch_table* table_for(uint32_t nw, bool versioned) {
 }
 
 bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+
+// Launches the two reclamation kernels: frees ring entries [head, end_pos)
+// (end_pos = 0 frees none) and returns parked spill slots to the free stack.
+mvbt_status launch_maintenance(mvbt_tree* t, hipStream_t s, unsigned long long end_pos) {
+  hipLaunchKernelGGL(reclaim_prep_kernel, dim3(1), dim3(64), 0, s, t->st, end_pos);
+  MVBT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(reclaim_copy_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->st,
+                     t->retire_ring, t->spill, t->free_stack);
+  MVBT_HIP(hipGetLastError());
+  if (!t->maint_ev) MVBT_HIP(hipEventCreateWithFlags(&t->maint_ev, hipEventDisableTiming));
+  MVBT_HIP(hipEventRecord(t->maint_ev, s));
+  t->spill_maybe = false;
+  return MVBT_OK;
+}
+
+bool other_writers_busy(mvbt_tree* t, hipStream_t s) {
+  for (auto& kv : t->write_ev)
+    if (kv.first != s && hipEventQuery(kv.second) == hipErrorNotReady) return true;
+  return false;
+}
+
+// Before a write kernel on stream s (caller holds t->mu).
+mvbt_status before_write(mvbt_tree* t, hipStream_t s) {
+  if (t->maint_ev) MVBT_HIP(hipStreamWaitEvent(s, t->maint_ev, 0));
+  if (t->spill_maybe && !other_writers_busy(t, s)) return launch_maintenance(t, s, 0);
+  return MVBT_OK;
+}
+
+// After a write kernel on stream s (caller holds t->mu).
+mvbt_status after_write(mvbt_tree* t, hipStream_t s) {
+  auto it = t->write_ev.find(s);
+  if (it == t->write_ev.end()) {
+    hipEvent_t e;
+    MVBT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    it = t->write_ev.emplace(s, e).first;
+  }
+  MVBT_HIP(hipEventRecord(it->second, s));
+  t->spill_maybe = true;
+  return MVBT_OK;
+}
 
 mvbt_status check_tree(mvbt_tree* t) {
   if (!t) return fail(MVBT_ERR_INVALID_ARG, "null tree handle");
@@ -315,6 +364,7 @@ mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
     if (t->nodes) (void)hipFree(t->nodes);
     if (t->st) (void)hipFree(t->st);
     if (t->free_stack) (void)hipFree(t->free_stack);
+    if (t->spill) (void)hipFree(t->spill);
     if (t->retire_ring) (void)hipFree(t->retire_ring);
     if (t->snap_rec) (void)hipHostFree(t->snap_rec);
   };
@@ -322,6 +372,7 @@ mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
   if (hipMalloc(&t->nodes, cfg.pool_nodes * node_bytes) != hipSuccess ||
       hipMalloc(&t->st, sizeof(tree_state)) != hipSuccess ||
       hipMalloc(&t->free_stack, cfg.pool_nodes * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&t->spill, cfg.pool_nodes * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&t->retire_ring, cfg.retire_capacity * sizeof(uint32_t)) != hipSuccess ||
       hipHostMalloc(&t->snap_rec, sizeof(snapshot_record), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
@@ -357,8 +408,11 @@ mvbt_status mvbt_destroy(mvbt_tree* t) {
   (void)hipFree(t->nodes);
   (void)hipFree(t->st);
   (void)hipFree(t->free_stack);
+  (void)hipFree(t->spill);
   (void)hipFree(t->retire_ring);
   (void)hipHostFree(t->snap_rec);
+  for (auto& kv : t->write_ev) (void)hipEventDestroy(kv.second);
+  if (t->maint_ev) (void)hipEventDestroy(t->maint_ev);
   delete t;
   return MVBT_OK;
 }
@@ -370,7 +424,11 @@ mvbt_status mvbt_insert(mvbt_tree* t, const uint32_t* keys, const uint32_t* valu
   if (n == 0) return MVBT_OK;
   if (!keys || !values) return fail(MVBT_ERR_INVALID_ARG, "null keys or values");
   device_guard g(t->device);
-  return table_for(t->nw, t->versioned)->insert(t, keys, values, n, (hipStream_t)stream);
+  std::lock_guard<std::mutex> lk(t->mu);
+  hipStream_t s = (hipStream_t)stream;
+  if (mvbt_status rc = before_write(t, s)) return rc;
+  if (mvbt_status rc = table_for(t->nw, t->versioned)->insert(t, keys, values, n, s)) return rc;
+  return after_write(t, s);
 }
 
 mvbt_status mvbt_erase(mvbt_tree* t, const uint32_t* keys, uint64_t n, void* stream, uint32_t flags) {
@@ -379,7 +437,11 @@ mvbt_status mvbt_erase(mvbt_tree* t, const uint32_t* keys, uint64_t n, void* str
   if (n == 0) return MVBT_OK;
   if (!keys) return fail(MVBT_ERR_INVALID_ARG, "null keys");
   device_guard g(t->device);
-  return table_for(t->nw, t->versioned)->erase(t, keys, nullptr, n, (hipStream_t)stream);
+  std::lock_guard<std::mutex> lk(t->mu);
+  hipStream_t s = (hipStream_t)stream;
+  if (mvbt_status rc = before_write(t, s)) return rc;
+  if (mvbt_status rc = table_for(t->nw, t->versioned)->erase(t, keys, nullptr, n, s)) return rc;
+  return after_write(t, s);
 }
 
 mvbt_status mvbt_find(mvbt_tree* t, const uint32_t* keys, uint32_t* values, uint64_t n,
@@ -456,12 +518,11 @@ mvbt_status mvbt_reclaim(mvbt_tree* t, void* stream) {
     if (t->live.count(b.first)) keep.push_back(b);
   t->bounds.swap(keep);
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(reclaim_prep_kernel, dim3(1), dim3(64), 0, s, t->st, end);
-  MVBT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(reclaim_copy_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->st,
-                     t->retire_ring, t->free_stack);
-  MVBT_HIP(hipGetLastError());
-  return MVBT_OK;
+  // Writers on other streams are ordered after this by maint_ev; writers
+  // already running on them must have finished (documented precondition).
+  for (auto& kv : t->write_ev)
+    if (kv.first != s) MVBT_HIP(hipStreamWaitEvent(s, kv.second, 0));
+  return launch_maintenance(t, s, end);
 }
 
 mvbt_status mvbt_memory_stats(mvbt_tree* t, mvbt_memstats* out) {
@@ -470,7 +531,7 @@ mvbt_status mvbt_memory_stats(mvbt_tree* t, mvbt_memstats* out) {
   device_guard g(t->device);
   tree_state s;
   if (mvbt_status rc = read_state(t, &s)) return rc;
-  uint64_t on_stack = s.free_top > 0 ? (uint64_t)s.free_top : 0;
+  uint64_t on_stack = (s.free_top > 0 ? (uint64_t)s.free_top : 0) + s.spill_count;
   uint64_t bump = std::min<uint64_t>(s.bump, s.capacity);
   out->node_bytes = (uint64_t)t->nw * 8;
   out->capacity_nodes = s.capacity;

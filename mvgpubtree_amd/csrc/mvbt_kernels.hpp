@@ -95,7 +95,10 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
                                                     const uint32_t* values, uint64_t n) {
   tile<NW> t;
   chunking<NW> c;
+  if (c.first >= n) return;
+  write_ctx ctx = tv.begin_writes(t);
   for (uint64_t base = c.first; base < n; base += c.stride) {
+    if (base != c.first) ctx.cur = tv.refresh_stamp(t);
     uint64_t i = base + t.rank();
     bool active = i < n;
     uint32_t key = active ? keys[i] : 0u;
@@ -103,10 +106,12 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
     uint64_t q = t.ballot(active);
     while (q) {
       uint32_t j = (uint32_t)lowest_bit(q);
+      ctx.want = (uint32_t)__builtin_popcountll(q);
       q &= q - 1;
-      tv.template write<OP>(t.shfl(key, j), OP == 0 ? t.shfl(val, j) : 0u, t);
+      tv.template write<OP>(t.shfl(key, j), OP == 0 ? t.shfl(val, j) : 0u, ctx, t);
     }
   }
+  tv.finish(ctx, t);
 }
 
 // take_snapshot_kernel (btree_kernels.hpp:473-478): one lane reads the retire
@@ -134,24 +139,30 @@ __global__ void reclaim_prep_kernel(tree_state* st, unsigned long long end_pos) 
   if (count > st->retire_cap) count = st->retire_cap;
   long long top = st->free_top;
   unsigned long long base = top > 0 ? (unsigned long long)top : 0ull;
+  unsigned long long spilled = st->spill_count;
   unsigned long long used = st->bump < st->capacity ? st->bump : st->capacity;
-  unsigned long long live = used - base;
+  unsigned long long live = used - base - spilled;
   if (live > st->peak_live) st->peak_live = live;
-  st->free_top = (long long)(base + count);
+  st->free_top = (long long)(base + count + spilled);
+  st->spill_count = 0;
   st->scratch[0] = head;  // first ring position to copy
   st->scratch[1] = count;
   st->scratch[2] = base;
+  st->scratch[3] = spilled;
   st->retire_head = end;
   st->reclaimed += count;
 }
 
-// Reclamation, phase 2: grid-stride copy ring -> free stack.
-__global__ void reclaim_copy_kernel(tree_state* st, const uint32_t* ring, uint32_t* free_stack) {
-  const unsigned long long start = st->scratch[0], count = st->scratch[1], base = st->scratch[2];
+// Reclamation, phase 2: grid-stride copy of the freed ring entries and the
+// parked spill slots onto the free stack.
+__global__ void reclaim_copy_kernel(tree_state* st, const uint32_t* ring, const uint32_t* spill,
+                                    uint32_t* free_stack) {
+  const unsigned long long start = st->scratch[0], count = st->scratch[1], base = st->scratch[2],
+                           spilled = st->scratch[3];
   const unsigned long long mask = st->retire_cap - 1;
-  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < count;
-       i += (unsigned long long)gridDim.x * blockDim.x) {
-    free_stack[base + i] = ring[(start + i) & mask];
+  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+       i < count + spilled; i += (unsigned long long)gridDim.x * blockDim.x) {
+    free_stack[base + i] = i < count ? ring[(start + i) & mask] : spill[i - count];
   }
 }
 

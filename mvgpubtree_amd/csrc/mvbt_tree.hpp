@@ -35,17 +35,28 @@
 
 namespace mvbt {
 
-// Device-resident state of one tree (one per handle).
-struct tree_state {
-  uint32_t root;          // index of the root node (fixed after creation)
-  uint32_t clock;         // write stamp; take_snapshot returns clock++
-  uint32_t error_flags;   // kErr* bits
-  uint32_t pad0;
-  unsigned long long bump;         // next never-used node index
-  unsigned long long capacity;     // pool capacity in nodes
-  long long free_top;              // recycled indices available in free_stack
+// Device-resident state of one tree (one per handle). Counters that many
+// tiles update are each on their own 128-byte line: a line shared by several
+// hot counters serialises every access to any of them.
+struct alignas(128) tree_state {
+  uint32_t root;           // index of the root node (fixed after creation)
+  uint32_t pad_root[31];
+  uint32_t clock;          // write stamp; take_snapshot returns clock++
+  uint32_t pad_clock[31];
+  unsigned long long bump;  // next never-used node index
+  unsigned long long pad_bump[15];
+  long long free_top;       // recycled indices available in free_stack
+  unsigned long long pad_free[15];
+  unsigned long long spill_count;  // unused magazine slots parked in `spill`
+  unsigned long long pad_spill[15];
   unsigned long long retire_tail;  // total retirements appended to the ring
+  unsigned long long pad_tail[15];
   unsigned long long retire_head;  // first retirement not yet reclaimed
+  unsigned long long pad_head[15];
+  uint32_t error_flags;            // kErr* bits
+  uint32_t pad_flags[31];
+  // cold
+  unsigned long long capacity;     // pool capacity in nodes
   unsigned long long retire_cap;   // ring capacity (power of two)
   unsigned long long reclaimed;    // total nodes returned to the free stack
   unsigned long long peak_live;    // max live nodes observed at reclaim/stats
@@ -60,6 +71,21 @@ struct snapshot_record {
   unsigned long long ring_pos;
 };
 
+// Per-tile writer context held in registers for the life of a tile's work:
+// an allocation magazine (lane i holds a free node index, lanes < mag_n valid)
+// and a retirement buffer (lanes < ret_n valid). One atomic on the shared
+// counters serves W allocations or W retirements. finish() must run before
+// the tile exits: it flushes retirements and parks unused slots in the spill
+// list, which the next reclamation returns to the free stack.
+struct write_ctx {
+  uint32_t mag;    // per lane
+  uint32_t ret;    // per lane
+  uint32_t mag_n;  // tile-uniform
+  uint32_t ret_n;  // tile-uniform
+  uint32_t cur;    // write stamp (tile-uniform)
+  uint32_t want;   // operations this tile still has queued (sizes refills)
+};
+
 template <int NW, bool VER>
 struct tree_view {
   using node_t = node<NW, VER>;
@@ -70,7 +96,9 @@ struct tree_view {
   uint64_t* nodes;         // pool: node i occupies nodes[i*NW .. i*NW+NW)
   tree_state* st;
   uint32_t* free_stack;    // recycled node indices, [0, free_top)
+  uint32_t* spill;         // parked unused magazine slots, [0, spill_count)
   uint32_t* retire_ring;   // retired node indices, ring of retire_cap
+  unsigned long long retire_cap;
   uint32_t root_;
   uint32_t cap_;           // pool capacity: every followed index must be below it
 
@@ -88,37 +116,103 @@ struct tree_view {
     else n.load_weak(at(idx), t);
   }
 
-  // ---- allocation (reference device_bump_allocator.hpp:45-54 and
-  // slab_alloc.hpp:131-205 roles). Pops a recycled index, else bumps the
-  // frontier. Returns kNone when the pool is exhausted.
-  MVBT_DEV uint32_t alloc(const tile_t& t) const {
-    uint32_t idx = kNone;
+  // ---- writer context --------------------------------------------------------
+  MVBT_DEV write_ctx begin_writes(const tile_t& t) const {
+    write_ctx c;
+    c.mag = kNone;
+    c.ret = kNone;
+    c.mag_n = 0;
+    c.ret_n = 0;
+    c.cur = refresh_stamp(t);
+    c.want = NW;
+    return c;
+  }
+  // Write stamp, read once per chunk of operations instead of once per op.
+  MVBT_DEV uint32_t refresh_stamp(const tile_t& t) const {
+    uint32_t c = 0;
+    if (VER && t.leader()) c = ld32_coherent(&st->clock);
+    return t.shfl(c, 0);
+  }
+
+  // ---- allocation (roles of the reference's device_bump_allocator.hpp:45-54
+  // and slab_alloc.hpp:131-205). Refills take W recycled indices from the free
+  // stack, the rest from the never-used frontier. Returns kNone when the pool
+  // is exhausted.
+  // Refill size: a versioned write copies its leaf once per epoch, so it needs
+  // about one slot per queued operation (+2 for splits); a plain write needs a
+  // slot only to split (2 for the root).
+  MVBT_DEV uint32_t refill_size(const write_ctx& c) const {
+    uint32_t k = VER ? c.want + 2 : 2;
+    return k < (uint32_t)NW ? k : (uint32_t)NW;
+  }
+  MVBT_DEV void refill(write_ctx& c, const tile_t& t) const {
+    const uint32_t k = refill_size(c);
+    long long top = 0;
+    unsigned long long b = 0;
     if (t.leader()) {
-      long long top = atomic_add_s64(&st->free_top, -1);
-      if (top > 0) {
-        idx = ld32_coherent(free_stack + (top - 1));
-      } else {
-        unsigned long long b = atomic_add64(&st->bump, 1ull);
-        if (b < st->capacity) idx = (uint32_t)b;
-        else atomic_or32(&st->error_flags, kErrPoolExhausted);
-      }
+      top = atomic_add_s64(&st->free_top, -(long long)k);
+      long long got = top < 0 ? 0 : (top > (long long)k ? (long long)k : top);
+      if (got < (long long)k) b = atomic_add64(&st->bump, (unsigned long long)(k - got));
     }
-    return t.shfl(idx, 0);
+    top = (long long)t.shfl64((uint64_t)top, 0);
+    b = t.shfl64(b, 0);
+    const uint32_t got = (uint32_t)(top < 0 ? 0 : (top > (long long)k ? (long long)k : top));
+    const uint32_t r = t.rank();
+    uint32_t idx = kNone;
+    if (r < got) {
+      idx = ld32_coherent(free_stack + (top - 1 - r));
+    } else if (r < k) {
+      unsigned long long f = b + (r - got);
+      idx = f < (unsigned long long)cap_ ? (uint32_t)f : kNone;
+    }
+    uint64_t ok = t.ballot(idx != kNone);
+    // valid lanes are a prefix: stack entries first, then fresh indices
+    c.mag = idx;
+    c.mag_n = (uint32_t)__builtin_popcountll(ok);
+    if (c.mag_n < k) flag(kErrPoolExhausted, t);
+  }
+  MVBT_DEV uint32_t alloc(write_ctx& c, const tile_t& t) const {
+    if (c.mag_n == 0) refill(c, t);
+    if (c.mag_n == 0) return kNone;
+    c.mag_n -= 1;
+    return t.shfl(c.mag, c.mag_n);
   }
   // Epoch retirement of a version copy (reference DEBR_device::retire,
   // memory_reclaimer.hpp:146-228). The copy becomes garbage once no snapshot
   // older than the epoch that created it is live.
-  MVBT_DEV void retire(uint32_t idx, const tile_t& t) const {
+  MVBT_DEV void flush_retired(write_ctx& c, const tile_t& t) const {
+    if (c.ret_n == 0) return;
+    unsigned long long pos = 0, head = 0;
     if (t.leader()) {
-      unsigned long long pos = atomic_add64(&st->retire_tail, 1ull);
-      unsigned long long head = __hip_atomic_load(&st->retire_head, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-      if (pos - head < st->retire_cap) {
-        st32_coherent(retire_ring + (pos & (st->retire_cap - 1)), idx);
-      } else {
-        // The slot still holds an unreclaimed entry: leave it; this copy leaks.
-        atomic_or32(&st->error_flags, kErrRetireOverflow);
-      }
+      pos = atomic_add64(&st->retire_tail, (unsigned long long)c.ret_n);
+      head = __hip_atomic_load(&st->retire_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    pos = t.shfl64(pos, 0);
+    head = t.shfl64(head, 0);
+    const uint32_t r = t.rank();
+    bool over = false;
+    if (r < c.ret_n) {
+      unsigned long long p = pos + r;
+      if (p - head < retire_cap) st32_coherent(retire_ring + (p & (retire_cap - 1)), c.ret);
+      else over = true;  // the slot still holds an unreclaimed entry: this copy leaks
+    }
+    if (t.any(over)) flag(kErrRetireOverflow, t);
+    c.ret_n = 0;
+  }
+  MVBT_DEV void retire(write_ctx& c, uint32_t idx, const tile_t& t) const {
+    if (t.rank() == c.ret_n) c.ret = idx;
+    c.ret_n += 1;
+    if (c.ret_n == (uint32_t)NW) flush_retired(c, t);
+  }
+  // Ends a tile's writes: flush retirements, park unused slots.
+  MVBT_DEV void finish(write_ctx& c, const tile_t& t) const {
+    flush_retired(c, t);
+    if (c.mag_n) {
+      unsigned long long s = 0;
+      if (t.leader()) s = atomic_add64(&st->spill_count, (unsigned long long)c.mag_n);
+      s = t.shfl64(s, 0);
+      if (t.rank() < c.mag_n) st32_coherent(spill + s + t.rank(), c.mag);
+      c.mag_n = 0;
     }
   }
 
@@ -129,12 +223,11 @@ struct tree_view {
     return (t.shfl(old, 0) & kLockBit) == 0;
   }
   MVBT_DEV void unlock(uint32_t idx, const tile_t& t) const {
-    fence_release();
+    order_release();
     if (t.leader()) atomic_and32(lock_word(idx), ~kLockBit);
   }
 
   // ---- snapshots ------------------------------------------------------------
-  MVBT_DEV uint32_t current_stamp() const { return ld32_coherent(&st->clock); }
   // Takes a snapshot: returns the id s; writes after this call are stamped > s.
   MVBT_DEV uint32_t take_snapshot(const tile_t& t) const {
     uint32_t s = 0;
@@ -262,7 +355,7 @@ struct tree_view {
   // Splits the full root in place: its pairs move to two new children and the
   // root keeps {pivot0 -> left, pivot -> right} (reference split_as_root,
   // versioned_node.hpp:185-212, with the root never a leaf here).
-  MVBT_DEV uint32_t split_root(const tile_t& t) const {
+  MVBT_DEV uint32_t split_root(write_ctx& c, const tile_t& t) const {
     uint32_t r = root_;
     if (!try_lock(r, t)) return kAtRoot;
     node_t n;
@@ -271,8 +364,8 @@ struct tree_view {
       unlock(r, t);
       return kAtRoot;
     }
-    uint32_t left = alloc(t);
-    uint32_t right = (left == kNone) ? kNone : alloc(t);
+    uint32_t left = alloc(c, t);
+    uint32_t right = (left == kNone) ? kNone : alloc(c, t);
     if (right == kNone) {
       unlock(r, t);
       return kOutOfMemory;
@@ -287,7 +380,7 @@ struct tree_view {
     if (VER) ln.set_lane(node_t::kVerLane, internal_ver, t);
     ln.store(at(left), t);
     rn.store(at(right), t);
-    fence_release();
+    order_release();
     uint32_t k0 = lo32(n.lane_pair(0, t));
     uint32_t rk = t.rank();
     if (node_t::data_lane(rk)) {
@@ -301,7 +394,7 @@ struct tree_view {
   // Splits full internal node `idx` whose parent is `parent` (reference
   // split, versioned_node.hpp:149-178; parent locked and re-checked as in
   // gpu_versioned_blink_tree.hpp:934-1009, but with try_lock and restart).
-  MVBT_DEV uint32_t split_internal(uint32_t idx, uint32_t parent, uint32_t key,
+  MVBT_DEV uint32_t split_internal(uint32_t idx, uint32_t parent, uint32_t key, write_ctx& c,
                                    const tile_t& t) const {
     if (!try_lock(idx, t)) return kAtParent;
     node_t n;
@@ -322,7 +415,7 @@ struct tree_view {
       unlock(idx, t);
       return kAtRoot;
     }
-    uint32_t y = alloc(t);
+    uint32_t y = alloc(c, t);
     if (y == kNone) {
       unlock(parent, t);
       unlock(idx, t);
@@ -331,10 +424,10 @@ struct tree_view {
     uint32_t pivot = n.split_pivot(t);
     node_t yn = n.upper_half(m, pack(0u, kNone), t);
     yn.store(at(y), t);
-    fence_release();
+    order_release();
     n.keep_lower_half(pivot, y, t);
     n.store(at(idx), t);  // metadata lane keeps the lock bit
-    fence_release();
+    order_release();
     pn.insert(pivot, y, t);
     pn.store_body(at(parent), t);
     unlock(parent, t);
@@ -345,19 +438,19 @@ struct tree_view {
   // Publishes a new epoch for locked leaf `idx` if its stamp is older than
   // `cur`: copies the current content out, publishes {cur, copy} in the
   // version word, fenced on both sides. Returns false when out of memory.
-  MVBT_DEV bool open_epoch(node_t& n, uint32_t idx, uint32_t cur, const tile_t& t) const {
+  MVBT_DEV bool open_epoch(node_t& n, uint32_t idx, write_ctx& c, const tile_t& t) const {
     if constexpr (VER) {
       uint64_t ver = n.version(t);
-      if (node_t::stamp(ver) != cur) {
-        uint32_t c = alloc(t);
-        if (c == kNone) return false;
-        n.store_unlocked_copy(at(c), t);
-        retire(c, t);
-        fence_release();
-        uint64_t nv = pack(cur, c);
+      if (node_t::stamp(ver) != c.cur) {
+        uint32_t cp = alloc(c, t);
+        if (cp == kNone) return false;
+        n.store_unlocked_copy(at(cp), t);
+        retire(c, cp, t);
+        order_release();
+        uint64_t nv = pack(c.cur, cp);
         if (t.leader()) st_coherent(at(idx) + node_t::kVerLane, nv);
         n.set_lane(node_t::kVerLane, nv, t);
-        fence_release();
+        order_release();
       }
     }
     return true;
@@ -365,7 +458,7 @@ struct tree_view {
 
   // Splits locked full leaf `idx` (loaded in `n`) under locked-and-checked
   // parent; always unlocks `idx`.
-  MVBT_DEV uint32_t split_leaf(node_t& n, uint32_t idx, uint32_t parent, uint32_t cur,
+  MVBT_DEV uint32_t split_leaf(node_t& n, uint32_t idx, uint32_t parent, write_ctx& c,
                                const tile_t& t) const {
     if (parent == kNone) {
       unlock(idx, t);
@@ -382,8 +475,8 @@ struct tree_view {
       unlock(idx, t);
       return kAtRoot;
     }
-    uint32_t y = alloc(t);
-    if (y == kNone || !open_epoch(n, idx, cur, t)) {
+    uint32_t y = alloc(c, t);
+    if (y == kNone || !open_epoch(n, idx, c, t)) {
       unlock(parent, t);
       unlock(idx, t);
       return kOutOfMemory;  // an allocated y is simply never linked; counted as leaked
@@ -393,14 +486,14 @@ struct tree_view {
     uint64_t ver = VER ? n.version(t) : 0ull;
     node_t yn = n.upper_half(m, ver, t);
     yn.store(at(y), t);
-    fence_release();
+    order_release();
     n.keep_lower_half(pivot, y, t);
     if constexpr (VER) {
       if (t.rank() != node_t::kVerLane) st_coherent(at(idx) + t.rank(), n.p);
     } else {
       n.store(at(idx), t);
     }
-    fence_release();
+    order_release();
     pn.insert(pivot, y, t);
     pn.store_body(at(parent), t);
     unlock(parent, t);
@@ -411,12 +504,11 @@ struct tree_view {
   // Shared writer loop. OP = 0 insert/update, 1 erase. Returns true if the
   // tree changed (insert: always when it succeeds; erase: key was present).
   template <int OP>
-  MVBT_DEV bool write(uint32_t key, uint32_t value, const tile_t& t) const {
+  MVBT_DEV bool write(uint32_t key, uint32_t value, write_ctx& c, const tile_t& t) const {
     if (!valid_key(key)) {
       flag(kErrInvalidKey, t);
       return false;
     }
-    const uint32_t cur = VER ? current_stamp() : 0u;
     uint32_t idx = root_, parent = kNone;
     node_t n;
     for (uint32_t step = 0;; ++step) {
@@ -424,6 +516,8 @@ struct tree_view {
         flag(kErrCorrupt, t);
         return false;
       }
+      // Coherent (L1-bypassing) read: a writer acts on what it reads (split
+      // decisions), and a stale L1 image of a hot node would repeat forever.
       n.load(at(idx), t);
       uint64_t m = n.meta(t);
       if (key >= node_t::high_key(m)) {
@@ -433,9 +527,9 @@ struct tree_view {
       }
       if (!node_t::is_leaf(m)) {
         if (OP == 0 && n.is_full(t)) {  // proactive split on the way down
-          uint32_t s = (idx == root_) ? split_root(t)
+          uint32_t s = (idx == root_) ? split_root(c, t)
                        : (parent == kNone) ? (uint32_t)kAtRoot
-                                           : split_internal(idx, parent, key, t);
+                                           : split_internal(idx, parent, key, c, t);
           if (s == kOutOfMemory) return false;
           if (s == kAtParent && parent != kNone) idx = parent;
           else idx = root_;
@@ -464,13 +558,13 @@ struct tree_view {
           return false;
         }
       } else if (n.is_full(t) && !n.contains(key, t)) {
-        uint32_t s = split_leaf(n, idx, parent, cur, t);
+        uint32_t s = split_leaf(n, idx, parent, c, t);
         if (s == kOutOfMemory) return false;
         idx = (s == kAtParent && parent != kNone) ? parent : root_;
         parent = kNone;
         continue;
       }
-      if (!open_epoch(n, idx, cur, t)) {
+      if (!open_epoch(n, idx, c, t)) {
         unlock(idx, t);
         return false;
       }
@@ -483,10 +577,12 @@ struct tree_view {
     }
   }
 
-  MVBT_DEV bool insert(uint32_t key, uint32_t value, const tile_t& t) const {
-    return write<0>(key, value, t);
+  MVBT_DEV bool insert(uint32_t key, uint32_t value, write_ctx& c, const tile_t& t) const {
+    return write<0>(key, value, c, t);
   }
-  MVBT_DEV bool erase(uint32_t key, const tile_t& t) const { return write<1>(key, 0u, t); }
+  MVBT_DEV bool erase(uint32_t key, write_ctx& c, const tile_t& t) const {
+    return write<1>(key, 0u, c, t);
+  }
 };
 
 }  // namespace mvbt
