@@ -236,6 +236,18 @@ def load_traffic(kernel: str):
         return None
 
 
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 5):
+    """Plain device-to-device copies of known size: under rocprofv3 --pmc
+    their FETCH_SIZE / WRITE_SIZE give the counters' bytes-per-unit scale."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,7 +257,12 @@ def main():
     ap.add_argument("--ops", type=int, default=1_000_000)
     ap.add_argument("--node-width", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-copy", action="store_true",
+                    help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.profile_copy:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the tree is a single-device structure here "
               "(multi-GPU key-range sharding is design-only); run with --gpus 1", file=sys.stderr)
