@@ -165,6 +165,44 @@ mvbt_status mvbt_validate(mvbt_tree* tree, uint64_t* num_keys, uint32_t* height)
  * `capacity` pairs; *count receives the number of keys. Synchronous. */
 mvbt_status mvbt_export(mvbt_tree* tree, mvbt_pair* out, uint64_t capacity, uint64_t* count);
 
+/* ---- Batched form: many batches of mixed operations in one launch ---------- */
+
+#define MVBT_OP_FIND 0u
+#define MVBT_OP_INSERT 1u
+#define MVBT_OP_ERASE 2u
+#define MVBT_OP_RANGE 3u
+
+/* One operation. key = key or range lower bound; arg = insert value or range
+ * upper bound (exclusive); slot = range output slot (pairs at pairs + slot*L). */
+typedef struct mvbt_op {
+  uint32_t type;
+  uint32_t key;
+  uint32_t arg;
+  uint32_t slot;
+} mvbt_op;
+
+#define MVBT_USE_GRAPH 0x2u /* launch through the handle's captured hipGraph */
+
+/* No single reference counterpart: it generalises the reference's mixed kernels
+ * concurrent_insert_range / concurrent_find_erase (gpu_versioned_blink_tree.hpp:218-294,
+ * btree_kernels.hpp:31-306) to any mix, deterministically. Runs num_batches batches
+ * (device array batch_offsets[num_batches+1], offsets[0] = 0, last = num_ops) in one
+ * persistent work-queue launch. Batch b reads (find, range) at snapshot base + b, i.e.
+ * it sees exactly the writes of the batches before it; identical to calling
+ * take_snapshot + find/range + insert/erase batch by batch. results[i]: find -> value
+ * or MVBT_NOT_FOUND; range -> true count (at most L pairs written); insert -> 1 new
+ * key, 0 value replaced; erase -> 1 erased, 0 absent; failed write -> 0xFFFFFFFF.
+ * The tree clock advances by num_batches; *snapshot_base (device pointer, may be
+ * NULL) receives base. Versioned trees only; scratch for num_batches is allocated
+ * on first use (synchronous then). */
+mvbt_status mvbt_run_batches(mvbt_tree* tree, const mvbt_op* ops, uint64_t num_ops,
+                             const uint64_t* batch_offsets, uint32_t num_batches,
+                             uint32_t* results, mvbt_pair* pairs, uint64_t pair_slots, uint32_t L,
+                             uint32_t* snapshot_base, void* stream, uint32_t flags);
+
+/* Worker occupancy of the batched form in waves per CU (autotuner knob). */
+mvbt_status mvbt_set_worker_waves(mvbt_tree* tree, uint32_t waves_per_cu);
+
 /* Tree parameters. */
 mvbt_status mvbt_node_width(mvbt_tree* tree, uint32_t* node_width);
 mvbt_status mvbt_kind_of(mvbt_tree* tree, uint32_t* kind);

@@ -31,6 +31,8 @@ LIB_PATH = os.path.join(PKG, "libmvbt.so")
 LATEST = 0xFFFFFFFF
 NOT_FOUND = 0xFFFFFFFF
 CONCURRENT = 0x1
+USE_GRAPH = 0x2
+OP_FIND, OP_INSERT, OP_ERASE, OP_RANGE = 0, 1, 2, 3
 FLAG_POOL_EXHAUSTED = 0x1
 FLAG_RETIRE_OVERFLOW = 0x2
 FLAG_INVALID_KEY = 0x4
@@ -46,7 +48,7 @@ EXPORTED_SYMBOLS = (
     "mvbt_last_error", "mvbt_create", "mvbt_destroy", "mvbt_insert", "mvbt_find", "mvbt_erase",
     "mvbt_range_query", "mvbt_take_snapshot", "mvbt_release_snapshot", "mvbt_reclaim",
     "mvbt_memory_stats", "mvbt_error_flags", "mvbt_clear_error_flags", "mvbt_validate",
-    "mvbt_export", "mvbt_node_width", "mvbt_kind_of",
+    "mvbt_export", "mvbt_node_width", "mvbt_kind_of", "mvbt_run_batches", "mvbt_set_worker_waves",
 )
 
 
@@ -128,6 +130,8 @@ def load_library():
         "mvbt_export": ([vp, vp, u64, u64p], i32),
         "mvbt_node_width": ([vp, u32p], i32),
         "mvbt_kind_of": ([vp, u32p], i32),
+        "mvbt_run_batches": ([vp, vp, u64, vp, u32, vp, vp, u64, u32, vp, vp, u32], i32),
+        "mvbt_set_worker_waves": ([vp, u32], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -293,3 +297,24 @@ class GpuVersionedBTree(_TreeBase):
 
     def reclaim(self, stream=None):
         _check(load_library().mvbt_reclaim(self._h, _stream(stream)), "reclaim")
+
+    def run_batches(self, ops, batch_offsets, results=None, pairs=None, pair_slots: int = 0,
+                    average_range_length: int = 0, snapshot_base=None, stream=None,
+                    use_graph: bool = False):
+        """Batched form: `ops` is an int32 device tensor of shape (n, 4) holding
+        {type, key, arg, slot} (OP_* types); `batch_offsets` an int64 device
+        tensor of num_batches + 1 op offsets. Returns the results tensor."""
+        torch = _torch()
+        n = ops.shape[0]
+        if results is None:
+            results = torch.empty(n, dtype=torch.int32, device=ops.device)
+        nb = batch_offsets.numel() - 1
+        _check(load_library().mvbt_run_batches(
+            self._h, _ptr(ops, 4, "ops"), n, _ptr(batch_offsets, 8, "batch_offsets"), nb,
+            _ptr(results, 4, "results"), _ptr(pairs, 8, "pairs"), pair_slots,
+            average_range_length, _ptr(snapshot_base, 4, "snapshot_base"), _stream(stream),
+            USE_GRAPH if use_graph else 0), "run_batches")
+        return results
+
+    def set_worker_waves(self, waves_per_cu: int):
+        _check(load_library().mvbt_set_worker_waves(self._h, waves_per_cu), "set_worker_waves")

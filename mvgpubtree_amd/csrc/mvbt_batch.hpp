@@ -1,0 +1,223 @@
+// Persistent work-queue form: many batches of mixed operations in one launch.
+//
+// Semantics (identical to issuing the batches one by one through the plain
+// per-batch API with a snapshot before each): batch b reads (find, range) at
+// snapshot clock0 + b, i.e. it sees exactly the writes of batches < b; its
+// writes (insert, erase) are stamped clock0 + b + 1. The launch reserves the
+// nb snapshot ids by advancing the tree clock by nb.
+//
+// Scheduling: tiles take chunks of W consecutive operations from one queue in
+// order. An operation of batch b may run once every write of batches < b has
+// completed (a device frontier F = last batch whose writes are all done). A
+// write of batch b must also wait for that, because leaf version chains are
+// stamp-ordered: a late write of batch b-1 must never follow a copy made by
+// batch b. Reads of batch b then overlap writes of batch b and later work.
+// The lowest unexecuted operation is always runnable, so the queue drains.
+// A tile that must wait polls F once per pass of its outer loop and never
+// spins inside a branch (the tiles of a wave run in lockstep).
+//
+// Reference counterpart: the ratio-partitioned mixed kernels
+// concurrent_insert_range_kernel / concurrent_find_erase_kernel_versioned
+// (btree_kernels.hpp:31-306), which take per-tile snapshots inside the
+// kernel and are therefore not deterministic.
+#pragma once
+#include "mvbt_tree.hpp"
+
+namespace mvbt {
+
+enum : uint32_t { kOpFind = 0, kOpInsert = 1, kOpErase = 2, kOpRange = 3 };
+
+// One operation: {type, key or range lower bound, value or range upper
+// bound, output slot of a range (pairs at pairs + slot * L)}.
+struct op_record {
+  uint32_t type, key, arg, slot;
+};
+
+struct alignas(128) batch_job {
+  // per-call parameters (written by batch_setup_kernel)
+  const op_record* ops;
+  unsigned long long nops;
+  const unsigned long long* offsets;  // nb + 1 op offsets, offsets[0] = 0
+  uint32_t nb;
+  uint32_t L;
+  uint32_t* results;     // one word per op
+  uint64_t* pairs;       // range output
+  unsigned long long pair_slots;  // capacity of `pairs` in slots of L pairs
+  uint32_t* base_out;    // optional: receives clock0
+  uint32_t clock0;
+  uint32_t pad0;
+  // library-owned scratch, capacity batches each
+  uint32_t* nwrites;
+  uint32_t* done;
+  int32_t* advance;
+  unsigned long long pad1[3];
+  // hot runtime words, one line each
+  unsigned long long next_op;
+  unsigned long long pad2[15];
+  int32_t frontier;
+  int32_t pad3[31];
+};
+
+struct batch_params {
+  const op_record* ops;
+  unsigned long long nops;
+  const unsigned long long* offsets;
+  uint32_t nb;
+  uint32_t L;
+  uint32_t* results;
+  uint64_t* pairs;
+  unsigned long long pair_slots;
+  uint32_t* base_out;
+};
+
+// 1. Parameters in, clock range reserved, queue reset.
+__global__ void batch_setup_kernel(batch_job* job, tree_state* st, batch_params p) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  job->ops = p.ops;
+  job->nops = p.nops;
+  job->offsets = p.offsets;
+  job->nb = p.nb;
+  job->L = p.L;
+  job->results = p.results;
+  job->pairs = p.pairs;
+  job->pair_slots = p.pair_slots;
+  job->base_out = p.base_out;
+  job->clock0 = __hip_atomic_fetch_add(&st->clock, p.nb, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  job->next_op = 0;
+  job->frontier = -1;
+}
+
+// 2. Writes per batch (one block per batch, grid-stride over batches).
+__global__ void batch_count_kernel(batch_job* job) {
+  __shared__ uint32_t partial[4];
+  const uint32_t nb = job->nb;
+  for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const unsigned long long lo = job->offsets[b], hi = job->offsets[b + 1];
+    uint32_t c = 0;
+    for (unsigned long long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      uint32_t ty = job->ops[i].type;
+      c += (ty == kOpInsert || ty == kOpErase) ? 1u : 0u;
+    }
+    // wave reduction via ballot-free shuffles
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+      for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) tot += partial[w];
+      job->nwrites[b] = tot;
+      job->done[b] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// 3. advance[b] = frontier value once batch b's writes are done: the last
+// batch of the run of write-free batches that follows b.
+__global__ void batch_frontier_kernel(batch_job* job) {
+  const uint32_t nb = job->nb;
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+    uint32_t j = b + 1;
+    while (j < nb && job->nwrites[j] == 0) ++j;
+    job->advance[b] = (int32_t)j - 1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    uint32_t j = 0;
+    while (j < nb && job->nwrites[j] == 0) ++j;
+    job->frontier = (int32_t)j - 1;
+  }
+}
+
+// 5. Hands clock0 back.
+__global__ void batch_finish_kernel(batch_job* job) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && job->base_out) *job->base_out = job->clock0;
+}
+
+MVBT_DEV uint32_t batch_of(const unsigned long long* offsets, uint32_t nb, unsigned long long i) {
+  uint32_t lo = 0, hi = nb;  // offsets[lo] <= i < offsets[hi]
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (offsets[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// 4. The persistent worker.
+template <int NW>
+__global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_job* job) {
+  const tree_view<NW, true> tv(tp);
+  tile<NW> t;
+  const unsigned long long nops = job->nops;
+  const uint32_t nb = job->nb, L = job->L, clock0 = job->clock0;
+  const op_record* ops = job->ops;
+  uint32_t* results = job->results;
+  uint64_t* pairs = job->pairs;
+  const unsigned long long pair_slots = job->pair_slots;
+  write_ctx ctx = tv.begin_writes(t);
+  int32_t seen = -1;  // last frontier value observed (tile-uniform)
+  unsigned long long base = 0;
+  uint32_t len = 0, pos = 0;
+  op_record mine{};
+  uint32_t my_batch = 0, my_result = 0;
+  uint32_t waits = 0;  // consecutive passes without progress
+  for (;;) {
+    if (pos == len) {
+      if (len && t.rank() < len) results[base + t.rank()] = my_result;
+      unsigned long long b0 = 0;
+      if (t.leader()) b0 = atomic_add64(&job->next_op, (unsigned long long)NW);
+      base = t.shfl64(b0, 0);
+      if (base >= nops) break;
+      len = (uint32_t)((nops - base) < (unsigned long long)NW ? (nops - base) : NW);
+      pos = 0;
+      if (t.rank() < len) {
+        mine = ops[base + t.rank()];
+        my_batch = batch_of(job->offsets, nb, base + t.rank());
+      }
+      my_result = kEmpty;
+    }
+    const uint32_t b = t.shfl(my_batch, pos);
+    if ((int32_t)b - 1 > seen) {
+      int32_t f = 0;
+      if (t.leader()) f = __hip_atomic_load(&job->frontier, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      seen = (int32_t)t.shfl((uint32_t)f, 0);
+      if ((int32_t)b - 1 > seen) {
+        if (++waits > (1u << 24)) {  // ~seconds: the frontier never moved
+          tv.flag(kErrCorrupt, t);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(4);
+        continue;
+      }
+    }
+    waits = 0;
+    const uint32_t ty = t.shfl(mine.type, pos), key = t.shfl(mine.key, pos);
+    const uint32_t arg = t.shfl(mine.arg, pos);
+    uint32_t r;
+    if (ty == kOpFind) {
+      r = tv.template find<true>(key, clock0 + b, t);
+    } else if (ty == kOpRange) {
+      const uint32_t slot = t.shfl(mine.slot, pos);
+      uint64_t* out = (pairs && slot < pair_slots) ? pairs + (size_t)slot * L : nullptr;
+      if (pairs && slot >= pair_slots) tv.flag(kErrCorrupt, t);
+      r = tv.template range<true>(key, arg, clock0 + b, out, L, t);
+      if (r > L) tv.flag(kErrRangeOverflow, t);
+    } else {
+      ctx.cur = clock0 + b + 1;
+      ctx.want = len - pos;
+      r = (ty == kOpInsert) ? tv.template write<0>(key, arg, ctx, t)
+                            : tv.template write<1>(key, 0u, ctx, t);
+      order_release();  // this write's stores complete before it is counted
+      if (t.leader()) {
+        uint32_t prev = atomic_add32(job->done + b, 1u);
+        if (prev + 1 == job->nwrites[b])
+          __hip_atomic_store(&job->frontier, job->advance[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (t.rank() == pos) my_result = r;
+    ++pos;
+  }
+  tv.finish(ctx, t);
+}
+
+}  // namespace mvbt

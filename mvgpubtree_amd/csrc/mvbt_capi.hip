@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/mvbt.h"
+#include "mvbt_batch.hpp"
 #include "mvbt_kernels.hpp"
 
 using namespace mvbt;
@@ -58,10 +59,19 @@ struct mvbt_tree {
   std::map<hipStream_t, hipEvent_t> write_ev;
   hipEvent_t maint_ev = nullptr;
   bool spill_maybe = false;
+  // Batched form: device job descriptor, per-batch scratch, graph.
+  batch_job* job = nullptr;
+  uint32_t* b_nwrites = nullptr;
+  uint32_t* b_done = nullptr;
+  int32_t* b_advance = nullptr;
+  uint32_t b_capacity = 0;
+  uint32_t worker_waves = 16;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraphNode_t graph_setup = nullptr;
 
-  template <int NW, bool VER>
-  tree_view<NW, VER> view() const {
-    tree_view<NW, VER> v;
+  tree_ptrs ptrs() const {
+    tree_ptrs v;
     v.nodes = nodes;
     v.st = st;
     v.free_stack = free_stack;
@@ -71,6 +81,10 @@ struct mvbt_tree {
     v.root_ = 0;
     v.cap_ = (uint32_t)cfg.pool_nodes;
     return v;
+  }
+  template <int NW, bool VER>
+  tree_view<NW, VER> view() const {
+    return tree_view<NW, VER>(ptrs());
   }
 };
 
@@ -141,6 +155,11 @@ struct impl {
   }
 };
 
+template <int NW>
+void* worker_fn() {
+  return reinterpret_cast<void*>(&batch_worker_kernel<NW>);
+}
+
 struct dispatch_table {
   mvbt_status (*init)(mvbt_tree*);
   mvbt_status (*find)(mvbt_tree*, const uint32_t*, uint32_t*, uint64_t, uint32_t, hipStream_t, bool);
@@ -148,12 +167,14 @@ struct dispatch_table {
                        uint64_t, uint32_t, hipStream_t, bool);
   mvbt_status (*insert)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, hipStream_t);
   mvbt_status (*erase)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, hipStream_t);
+  void* (*worker)();
 };
 
 template <int NW, bool VER>
 dispatch_table make_table() {
   using I = impl<NW, VER>;
-  return dispatch_table{&I::init, &I::find, &I::range, &I::template write<0>, &I::template write<1>};
+  return dispatch_table{&I::init, &I::find, &I::range, &I::template write<0>, &I::template write<1>,
+                        VER ? &worker_fn<NW> : nullptr};
 }
 
 const dispatch_table* table_for(uint32_t nw, bool versioned) {
@@ -325,6 +346,83 @@ mvbt_status walk_tree(const host_walk& w, std::vector<mvbt_pair>* pairs, uint64_
   return MVBT_OK;
 }
 
+// ---- batched form helpers ----------------------------------------------------
+
+mvbt_status ensure_batch_scratch(mvbt_tree* t, uint32_t nb) {
+  if (!t->job) {
+    MVBT_HIP(hipMalloc(&t->job, sizeof(batch_job)));
+    MVBT_HIP(hipMemset(t->job, 0, sizeof(batch_job)));
+  }
+  if (nb <= t->b_capacity) return MVBT_OK;
+  uint32_t cap = 1024;
+  while (cap < nb) cap *= 2;
+  MVBT_HIP(hipDeviceSynchronize());
+  if (t->b_nwrites) (void)hipFree(t->b_nwrites);
+  if (t->b_done) (void)hipFree(t->b_done);
+  if (t->b_advance) (void)hipFree(t->b_advance);
+  t->b_nwrites = nullptr;
+  t->b_done = nullptr;
+  t->b_advance = nullptr;
+  t->b_capacity = 0;
+  MVBT_HIP(hipMalloc(&t->b_nwrites, cap * sizeof(uint32_t)));
+  MVBT_HIP(hipMalloc(&t->b_done, cap * sizeof(uint32_t)));
+  MVBT_HIP(hipMalloc(&t->b_advance, cap * sizeof(int32_t)));
+  // The scratch pointers live in the device descriptor; per-call parameters do not.
+  uint32_t* ptrs[3] = {t->b_nwrites, t->b_done, reinterpret_cast<uint32_t*>(t->b_advance)};
+  MVBT_HIP(hipMemcpy(&t->job->nwrites, ptrs, sizeof(ptrs), hipMemcpyHostToDevice));
+  t->b_capacity = cap;
+  return MVBT_OK;
+}
+
+uint32_t worker_blocks(const mvbt_tree* t) {
+  uint64_t b = (uint64_t)t->num_cus * t->worker_waves * 64 / kBlock;
+  return (uint32_t)std::max<uint64_t>(b, 1);
+}
+
+void destroy_graph(mvbt_tree* t) {
+  if (t->graph_exec) (void)hipGraphExecDestroy(t->graph_exec);
+  if (t->graph) (void)hipGraphDestroy(t->graph);
+  t->graph_exec = nullptr;
+  t->graph = nullptr;
+  t->graph_setup = nullptr;
+}
+
+// Explicit five-node graph: setup -> count -> frontier -> worker -> finish.
+// Every node but setup reads its inputs from the device descriptor, so a call
+// only updates the setup node's parameters.
+mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
+  destroy_graph(t);
+  MVBT_HIP(hipGraphCreate(&t->graph, 0));
+  batch_job* job = t->job;
+  tree_state* st = t->st;
+  batch_params p = bp;
+  void* setup_args[] = {&job, &st, &p};
+  void* job_args[] = {&job};
+  hipKernelNodeParams kp{};
+  hipGraphNode_t prev = nullptr, node = nullptr;
+  auto add = [&](void* fn, unsigned grid, unsigned block, void** args) -> hipError_t {
+    kp.func = fn;
+    kp.gridDim = dim3(grid);
+    kp.blockDim = dim3(block);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    hipError_t e = hipGraphAddKernelNode(&node, t->graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
+    prev = node;
+    return e;
+  };
+  MVBT_HIP(add(reinterpret_cast<void*>(&batch_setup_kernel), 1, 64, setup_args));
+  t->graph_setup = node;
+  MVBT_HIP(add(reinterpret_cast<void*>(&batch_count_kernel), t->num_cus * 4, kBlock, job_args));
+  MVBT_HIP(add(reinterpret_cast<void*>(&batch_frontier_kernel), t->num_cus, kBlock, job_args));
+  tree_ptrs tp = t->ptrs();
+  void* worker_args[] = {&tp, &job};
+  MVBT_HIP(add(table_for(t->nw, true)->worker(), worker_blocks(t), kBlock, worker_args));
+  MVBT_HIP(add(reinterpret_cast<void*>(&batch_finish_kernel), 1, 64, job_args));
+  MVBT_HIP(hipGraphInstantiate(&t->graph_exec, t->graph, nullptr, nullptr, 0));
+  return MVBT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -413,6 +511,11 @@ mvbt_status mvbt_destroy(mvbt_tree* t) {
   (void)hipHostFree(t->snap_rec);
   for (auto& kv : t->write_ev) (void)hipEventDestroy(kv.second);
   if (t->maint_ev) (void)hipEventDestroy(t->maint_ev);
+  destroy_graph(t);
+  if (t->job) (void)hipFree(t->job);
+  if (t->b_nwrites) (void)hipFree(t->b_nwrites);
+  if (t->b_done) (void)hipFree(t->b_done);
+  if (t->b_advance) (void)hipFree(t->b_advance);
   delete t;
   return MVBT_OK;
 }
@@ -581,6 +684,73 @@ mvbt_status mvbt_export(mvbt_tree* t, mvbt_pair* out, uint64_t capacity, uint64_
   if (mvbt_status rc = walk_tree(w, &pairs, nullptr, nullptr)) return rc;
   *count = pairs.size();
   if (out) std::memcpy(out, pairs.data(), std::min<uint64_t>(capacity, pairs.size()) * sizeof(mvbt_pair));
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
+                             const uint64_t* batch_offsets, uint32_t num_batches, uint32_t* results,
+                             mvbt_pair* pairs, uint64_t pair_slots, uint32_t L, uint32_t* snapshot_base,
+                             void* stream, uint32_t flags) {
+  static_assert(sizeof(mvbt_op) == sizeof(op_record), "op layout");
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!t->versioned) return fail(MVBT_ERR_UNSUPPORTED, "the batched form needs a versioned tree");
+  if (num_ops == 0 || num_batches == 0) return MVBT_OK;
+  if (!ops || !batch_offsets || !results) return fail(MVBT_ERR_INVALID_ARG, "null ops, offsets or results");
+  if (pairs && L == 0) return fail(MVBT_ERR_INVALID_ARG, "L must be > 0 when pairs is given");
+  device_guard g(t->device);
+  std::lock_guard<std::mutex> lk(t->mu);
+  if (mvbt_status rc = ensure_batch_scratch(t, num_batches)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (mvbt_status rc = before_write(t, s)) return rc;
+  batch_params p{};
+  p.ops = reinterpret_cast<const op_record*>(ops);
+  p.nops = num_ops;
+  p.offsets = reinterpret_cast<const unsigned long long*>(batch_offsets);
+  p.nb = num_batches;
+  p.L = L;
+  p.results = results;
+  p.pairs = reinterpret_cast<uint64_t*>(pairs);
+  p.pair_slots = pairs ? pair_slots : 0;
+  p.base_out = snapshot_base;
+  if (flags & MVBT_USE_GRAPH) {
+    if (!t->graph_exec) {
+      if (mvbt_status rc = build_graph(t, p)) return rc;
+    }
+    batch_job* job = t->job;
+    tree_state* st = t->st;
+    void* args[] = {&job, &st, &p};
+    hipKernelNodeParams kp{};
+    kp.func = reinterpret_cast<void*>(&batch_setup_kernel);
+    kp.gridDim = dim3(1);
+    kp.blockDim = dim3(64);
+    kp.kernelParams = args;
+    MVBT_HIP(hipGraphExecKernelNodeSetParams(t->graph_exec, t->graph_setup, &kp));
+    MVBT_HIP(hipGraphLaunch(t->graph_exec, s));
+  } else {
+    hipLaunchKernelGGL(batch_setup_kernel, dim3(1), dim3(64), 0, s, t->job, t->st, p);
+    hipLaunchKernelGGL(batch_count_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job);
+    hipLaunchKernelGGL(batch_frontier_kernel, dim3(t->num_cus), dim3(kBlock), 0, s, t->job);
+    tree_ptrs tp = t->ptrs();
+    batch_job* job = t->job;
+    void* wargs[] = {&tp, &job};
+    MVBT_HIP(hipLaunchKernel(table_for(t->nw, true)->worker(), dim3(worker_blocks(t)), dim3(kBlock),
+                             wargs, 0, s));
+    hipLaunchKernelGGL(batch_finish_kernel, dim3(1), dim3(64), 0, s, t->job);
+  }
+  MVBT_HIP(hipGetLastError());
+  return after_write(t, s);
+}
+
+mvbt_status mvbt_set_worker_waves(mvbt_tree* t, uint32_t waves_per_cu) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (waves_per_cu < 1 || waves_per_cu > 64) return fail(MVBT_ERR_INVALID_ARG, "waves_per_cu in [1, 64]");
+  std::lock_guard<std::mutex> lk(t->mu);
+  if (t->worker_waves != waves_per_cu) {
+    device_guard g(t->device);
+    (void)hipDeviceSynchronize();
+    destroy_graph(t);
+    t->worker_waves = waves_per_cu;
+  }
   return MVBT_OK;
 }
 

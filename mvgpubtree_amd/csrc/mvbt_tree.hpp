@@ -86,13 +86,9 @@ struct write_ctx {
   uint32_t want;   // operations this tile still has queued (sizes refills)
 };
 
-template <int NW, bool VER>
-struct tree_view {
-  using node_t = node<NW, VER>;
-  using tile_t = tile<NW>;
-  static constexpr int kWidth = NW;
-  static constexpr bool kVersioned = VER;
-
+// Device pointers of one tree, independent of the node width (passed to
+// kernels by value; tree_view adds the width-specific operations).
+struct tree_ptrs {
   uint64_t* nodes;         // pool: node i occupies nodes[i*NW .. i*NW+NW)
   tree_state* st;
   uint32_t* free_stack;    // recycled node indices, [0, free_top)
@@ -101,6 +97,17 @@ struct tree_view {
   unsigned long long retire_cap;
   uint32_t root_;
   uint32_t cap_;           // pool capacity: every followed index must be below it
+};
+
+template <int NW, bool VER>
+struct tree_view : tree_ptrs {
+  using node_t = node<NW, VER>;
+  using tile_t = tile<NW>;
+  static constexpr int kWidth = NW;
+  static constexpr bool kVersioned = VER;
+
+  MVBT_HD tree_view() = default;
+  MVBT_HD explicit tree_view(const tree_ptrs& p) : tree_ptrs(p) {}
 
   MVBT_DEV uint64_t* at(uint32_t idx) const { return nodes + (size_t)idx * NW; }
   MVBT_DEV uint32_t* lock_word(uint32_t idx) const {
@@ -351,6 +358,7 @@ struct tree_view {
 
   // ---- writers ----------------------------------------------------------------
   enum step : uint32_t { kAtRoot = 0, kAtParent = 1, kOutOfMemory = 2 };
+  static constexpr uint32_t kWriteSame = 0, kWriteNew = 1, kWriteFailed = kEmpty;
 
   // Splits the full root in place: its pairs move to two new children and the
   // root keeps {pivot0 -> left, pivot -> right} (reference split_as_root,
@@ -501,20 +509,22 @@ struct tree_view {
     return kAtParent;
   }
 
-  // Shared writer loop. OP = 0 insert/update, 1 erase. Returns true if the
-  // tree changed (insert: always when it succeeds; erase: key was present).
+  // Shared writer loop. OP = 0 insert/update, 1 erase. Returns kWriteNew
+  // (insert of a new key / erase of a present key), kWriteSame (insert that
+  // replaced a value / erase of an absent key) or kWriteFailed (reserved key,
+  // pool exhausted, corrupted structure; see the error flags).
   template <int OP>
-  MVBT_DEV bool write(uint32_t key, uint32_t value, write_ctx& c, const tile_t& t) const {
+  MVBT_DEV uint32_t write(uint32_t key, uint32_t value, write_ctx& c, const tile_t& t) const {
     if (!valid_key(key)) {
       flag(kErrInvalidKey, t);
-      return false;
+      return kWriteFailed;
     }
     uint32_t idx = root_, parent = kNone;
     node_t n;
     for (uint32_t step = 0;; ++step) {
       if (idx >= cap_ || step >= kMaxWriteSteps) {  // no lock is held here
         flag(kErrCorrupt, t);
-        return false;
+        return kWriteFailed;
       }
       // Coherent (L1-bypassing) read: a writer acts on what it reads (split
       // decisions), and a stale L1 image of a hot node would repeat forever.
@@ -530,7 +540,7 @@ struct tree_view {
           uint32_t s = (idx == root_) ? split_root(c, t)
                        : (parent == kNone) ? (uint32_t)kAtRoot
                                            : split_internal(idx, parent, key, c, t);
-          if (s == kOutOfMemory) return false;
+          if (s == kOutOfMemory) return kWriteFailed;
           if (s == kAtParent && parent != kNone) idx = parent;
           else idx = root_;
           parent = kNone;
@@ -555,32 +565,33 @@ struct tree_view {
       if (OP == 1) {
         if (!n.contains(key, t)) {
           unlock(idx, t);
-          return false;
+          return kWriteSame;
         }
       } else if (n.is_full(t) && !n.contains(key, t)) {
         uint32_t s = split_leaf(n, idx, parent, c, t);
-        if (s == kOutOfMemory) return false;
+        if (s == kOutOfMemory) return kWriteFailed;
         idx = (s == kAtParent && parent != kNone) ? parent : root_;
         parent = kNone;
         continue;
       }
       if (!open_epoch(n, idx, c, t)) {
         unlock(idx, t);
-        return false;
+        return kWriteFailed;
       }
-      if (OP == 0) n.insert(key, value, t);
+      bool fresh = true;
+      if (OP == 0) fresh = n.insert(key, value, t);
       else n.erase(key, t);
       uint32_t rk = t.rank();
       if (node_t::data_lane(rk)) st_coherent(at(idx) + rk, n.p);
       unlock(idx, t);
-      return true;
+      return fresh ? kWriteNew : kWriteSame;
     }
   }
 
-  MVBT_DEV bool insert(uint32_t key, uint32_t value, write_ctx& c, const tile_t& t) const {
+  MVBT_DEV uint32_t insert(uint32_t key, uint32_t value, write_ctx& c, const tile_t& t) const {
     return write<0>(key, value, c, t);
   }
-  MVBT_DEV bool erase(uint32_t key, write_ctx& c, const tile_t& t) const {
+  MVBT_DEV uint32_t erase(uint32_t key, write_ctx& c, const tile_t& t) const {
     return write<1>(key, 0u, c, t);
   }
 };

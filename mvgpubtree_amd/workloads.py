@@ -188,3 +188,37 @@ def config4(seed: int = 4, n_initial: int = 40_000_000, num_batches: int = 4096,
         batches.append(Batch(insert_keys=ik, insert_values=value_of(ik), erase_keys=ek,
                              find_keys=fk, range_lo=lo, range_hi=hi))
     return MixedWorkload(initial, value_of(initial), batches, range_capacity)
+
+
+def to_ops(batches, seed: int = 0, shuffle: bool = True):
+    """Flattens batches into the batched-form op stream.
+
+    Returns (ops uint32[n, 4] rows {type, key, arg, slot}, offsets uint64[nb+1],
+    number of range slots). Operations of different types are interleaved
+    inside each batch (a batch's semantics do not depend on order: reads see
+    the pre-batch state and the batch's writes touch distinct keys)."""
+    rng = np.random.default_rng(seed)
+    rows, offsets, slot = [], [0], 0
+    for b in batches:
+        parts = []
+        if b.find_keys.size:
+            z = np.zeros(b.find_keys.size, np.uint32)
+            parts.append(np.stack([z + OP_FIND, b.find_keys, z, z], 1))
+        if b.insert_keys.size:
+            z = np.zeros(b.insert_keys.size, np.uint32)
+            parts.append(np.stack([z + OP_INSERT, b.insert_keys, b.insert_values, z], 1))
+        if b.erase_keys.size:
+            z = np.zeros(b.erase_keys.size, np.uint32)
+            parts.append(np.stack([z + OP_ERASE, b.erase_keys, z, z], 1))
+        if b.range_lo.size:
+            z = np.zeros(b.range_lo.size, np.uint32)
+            slots = np.arange(slot, slot + b.range_lo.size, dtype=np.uint32)
+            slot += b.range_lo.size
+            parts.append(np.stack([z + OP_RANGE, b.range_lo, b.range_hi, slots], 1))
+        blk = np.concatenate(parts).astype(np.uint32) if parts else np.zeros((0, 4), np.uint32)
+        if shuffle:
+            blk = blk[rng.permutation(blk.shape[0])]
+        rows.append(blk)
+        offsets.append(offsets[-1] + blk.shape[0])
+    ops = np.ascontiguousarray(np.concatenate(rows)) if rows else np.zeros((0, 4), np.uint32)
+    return ops, np.array(offsets, dtype=np.uint64), slot

@@ -22,7 +22,7 @@ import subprocess
 
 import numpy as np
 
-from .sorted_ref import SortedState, range_bounds  # noqa: F401
+from .sorted_ref import SortedState, batch_expected, range_bounds  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")

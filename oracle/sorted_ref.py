@@ -99,3 +99,38 @@ class SortedState:
         self.keys = np.delete(self.keys, idx[hit])
         self.values = np.delete(self.values, idx[hit])
         return int(hit.sum())
+
+
+def batch_expected(initial_keys, initial_values, ops: np.ndarray, offsets: np.ndarray, L: int,
+                   num_slots: int):
+    """Expected results of the batched form: per batch, every read sees the
+    state before the batch; then the batch's writes apply. Returns
+    (results u32[n], pairs u64[num_slots*L] with empty slots all-ones)."""
+    st = SortedState(initial_keys, initial_values)
+    n = ops.shape[0]
+    res = np.zeros(n, dtype=np.uint32)
+    pairs = np.full(num_slots * L, np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+    for b in range(offsets.size - 1):
+        lo, hi = int(offsets[b]), int(offsets[b + 1])
+        blk = ops[lo:hi]
+        ty = blk[:, 0]
+        idx = np.arange(lo, hi)
+        f = ty == 0
+        res[idx[f]] = st.find(blk[f, 1])
+        r = ty == 3
+        if r.any():
+            p, c = st.range_query(blk[r, 1], blk[r, 2], L)
+            res[idx[r]] = c
+            slots = blk[r, 3].astype(np.int64)
+            pairs.reshape(-1, L)[slots] = p.reshape(-1, L)
+        ins = ty == 1
+        er = ty == 2
+        if ins.any():
+            res[idx[ins]] = (st.find(blk[ins, 1]) == EMPTY).astype(np.uint32)
+        if er.any():
+            res[idx[er]] = (st.find(blk[er, 1]) != EMPTY).astype(np.uint32)
+        if ins.any():
+            st.insert(blk[ins, 1], blk[ins, 2])
+        if er.any():
+            st.erase(blk[er, 1])
+    return res, pairs, st

@@ -1,0 +1,105 @@
+"""GPU parity tests of the persistent batched form (mvbt_run_batches) and of
+the plain per-batch path on the same mixed workloads (config-4 shape, scaled
+down): both bit-exact against the oracle, which answers every read of a batch
+from the state before the batch."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import mvgpubtree_amd as mv
+import oracle
+from mvgpubtree_amd import drivers
+from mvgpubtree_amd import workloads as wl
+
+from conftest import to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+
+def small_config4(seed, n_initial=200_000, num_batches=120, max_batch=8192):
+    return wl.config4(seed=seed, n_initial=n_initial, num_batches=num_batches, min_batch=32,
+                      max_batch=max_batch, range_capacity=256)
+
+
+def build_tree(w, dev, nw=16):
+    tree = mv.GpuVersionedBTree(pool_nodes=1 << 19, node_width=nw)
+    tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
+    return tree
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("nw", [16, 32, 64])
+def test_batched_form_matches_oracle(dev, nw, use_graph):
+    w = small_config4(seed=40 + nw)
+    tree = build_tree(w, dev, nw)
+    r = drivers.BatchedRunner(w.batches, w.range_capacity, dev, seed=nw)
+    s0 = tree.take_snapshot()
+    tree.release_snapshot(s0)
+    res = to_np(r.run(tree, use_graph=use_graph))
+    want, wpairs, final = oracle.batch_expected(w.initial_keys, w.initial_values, r.ops_host,
+                                                r.offsets_host, w.range_capacity, r.slots)
+    assert int(to_np(r.base)[0]) == s0 + 1
+    bad = np.nonzero(res != want)[0]
+    assert bad.size == 0, (bad[:10], r.ops_host[bad[:10]], res[bad[:10]], want[bad[:10]])
+    assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
+    # clock advanced by one snapshot per batch
+    assert tree.take_snapshot() == s0 + 1 + len(w.batches)
+    exp = tree.export()
+    assert np.array_equal(exp, final.keys.astype(np.uint64) | (final.values.astype(np.uint64) << np.uint64(32)))
+    assert tree.error_flags() == 0
+    tree.close()
+
+
+def test_per_batch_path_matches_oracle(dev):
+    w = small_config4(seed=7, num_batches=60)
+    tree = build_tree(w, dev)
+    r = drivers.PerBatchRunner(w.batches, w.range_capacity, dev)
+    got = r.run(tree, collect=True)
+    st = oracle.SortedState(w.initial_keys, w.initial_values)
+    for (f, c, p), b in zip(got, w.batches):
+        assert np.array_equal(f, st.find(b.find_keys))
+        wp, wc = st.range_query(b.range_lo, b.range_hi, w.range_capacity)
+        assert np.array_equal(c, wc)
+        assert np.array_equal(p, wp)
+        st.insert(b.insert_keys, b.insert_values)
+        st.erase(b.erase_keys)
+    assert tree.validate_tree_structure()[0] == len(st)
+    assert tree.error_flags() == 0
+    tree.close()
+
+
+def test_graph_replays_with_new_inputs(dev):
+    """Two launches through one captured graph (the second smaller, with new
+    parameters), then a direct launch: state carries across all three."""
+    w = small_config4(seed=9, num_batches=90)
+    parts = [w.batches[:50], w.batches[50:70], w.batches[70:]]
+    tree = build_tree(w, dev)
+    st_keys, st_vals = w.initial_keys, w.initial_values
+    for i, part in enumerate(parts):
+        r = drivers.BatchedRunner(part, w.range_capacity, dev, seed=i)
+        res = to_np(r.run(tree, use_graph=(i < 2)))
+        want, wpairs, final = oracle.batch_expected(st_keys, st_vals, r.ops_host, r.offsets_host,
+                                                    w.range_capacity, r.slots)
+        assert np.array_equal(res, want), f"part {i}"
+        assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
+        st_keys, st_vals = final.keys, final.values
+        tree.reclaim()
+    assert tree.validate_tree_structure()[0] == st_keys.size
+    assert tree.memory_stats().retired_pending == 0
+    assert tree.error_flags() == 0
+    tree.close()
+
+
+def test_batched_snapshot_reads_older_than_the_run(dev):
+    """A snapshot taken before a batched run still reads its own state."""
+    w = small_config4(seed=11, num_batches=40)
+    tree = build_tree(w, dev)
+    s = tree.take_snapshot()
+    r = drivers.BatchedRunner(w.batches, w.range_capacity, dev)
+    r.run(tree)
+    q = np.concatenate([w.initial_keys[:50000], np.concatenate([b.insert_keys for b in w.batches])])
+    got = to_np(tree.find(to_dev(q, dev), timestamp=s))
+    assert np.array_equal(got, oracle.SortedState(w.initial_keys, w.initial_values).find(q))
+    assert tree.error_flags() == 0
+    tree.close()
