@@ -225,6 +225,174 @@ def run_config2(args):
     return result
 
 
+def run_config4(args):
+    """Batched form (config 4): 4096 batches, sizes log-uniform in [32, 65536],
+    mix 60/20/10/10 find/insert/erase/range (8-128 keys), on a 40M-key tree.
+    The persistent work-queue launch (captured hipGraph, autotuned node width
+    and waves/CU) against the plain per-batch launch path on an identical tree."""
+    import torch
+
+    import mvgpubtree_amd as mv
+    from mvgpubtree_amd import autotune, drivers
+    from mvgpubtree_amd import workloads as wl
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    w = wl.config4(seed=4, n_initial=args.keys, num_batches=args.batches)
+    log(f"config4 workload: {sum(b.num_ops for b in w.batches)} ops in {len(w.batches)} batches "
+        f"({time.perf_counter() - t0:.1f}s)")
+    L = w.range_capacity
+    waves = tuple(int(x) for x in args.waves.split(","))
+    widths = tuple(int(x) for x in args.widths.split(","))
+    if len(widths) * len(waves) > 1:
+        tw = wl.config4(seed=44, n_initial=args.keys, num_batches=2 * len(waves) * args.tune_batches)
+        groups = [tw.batches[i::2 * len(waves)] for i in range(2 * len(waves))]
+        best, allr = autotune.tune_batched(tw.initial_keys, tw.initial_values, groups, L, dev,
+                                           node_widths=widths, waves=waves, log=log)
+        del tw
+    else:
+        best, allr = autotune.TuneResult(widths[0], waves[0], 0.0, 0.0), []
+    log(f"autotune best: node width {best.node_width}, {best.waves_per_cu} waves/CU")
+    n_ops = sum(b.num_ops for b in w.batches)
+    pool = int(args.keys / 4 * (16 / best.node_width) + 2 * n_ops) + (1 << 20)
+    dk, dv = dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev)
+
+    def fresh_tree():
+        t = mv.GpuVersionedBTree(pool_nodes=pool, node_width=best.node_width, retire_capacity=1 << 24)
+        t.insert(dk, dv)
+        t.set_worker_waves(best.waves_per_cu)
+        return t
+
+    # batched form: warm the graph on a scratch run, then one timed launch
+    br = drivers.BatchedRunner(w.batches, L, dev)
+    wb = wl.config4(seed=99, n_initial=1_000_000, num_batches=64)
+    scratch = mv.GpuVersionedBTree(pool_nodes=1 << 21, node_width=best.node_width)
+    scratch.insert(dev_tensor(wb.initial_keys, dev), dev_tensor(wb.initial_values, dev))
+    scratch.set_worker_waves(best.waves_per_cu)
+    drivers.BatchedRunner(wb.batches, L, dev).run(scratch, use_graph=True)
+    drivers.PerBatchRunner(wb.batches[:8], L, dev).run(scratch)
+    torch.cuda.synchronize()
+    scratch.close()
+
+    tree = fresh_tree()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    br.run(tree, use_graph=True)
+    torch.cuda.synchronize()
+    batched_s = time.perf_counter() - t0
+    flags = tree.error_flags()
+    res_b = br.results.cpu().numpy().view(np.uint32)
+    tree.close()
+
+    pr = drivers.PerBatchRunner(w.batches, L, dev)
+    tree = fresh_tree()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pr.run(tree)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    flags |= tree.error_flags()
+    tree.close()
+    if flags:
+        raise RuntimeError(f"device error flags 0x{flags:x}")
+    fails = int((res_b == 0xFFFFFFFF)[br.ops_host[:, 0] != wl.OP_FIND].sum())
+    return {
+        "batched_Mops": round(n_ops / batched_s / 1e6, 2),
+        "plain_per_batch_Mops": round(n_ops / plain_s / 1e6, 2),
+        "speedup": round(plain_s / batched_s, 2),
+        "batched_ms": round(batched_s * 1e3, 2), "plain_ms": round(plain_s * 1e3, 2),
+        "ops": n_ops, "batches": len(w.batches), "failed_writes": fails,
+        "tuned": {"node_width": best.node_width, "waves_per_cu": best.waves_per_cu},
+        "tuning": [{"nw": r.node_width, "waves": r.waves_per_cu, "Mops": round(r.mops, 1)} for r in allr],
+    }
+
+
+def gpu_unique_keys(n: int, extra: int, seed: int, dev):
+    """n + extra distinct uniform keys in [1, 2^32 - 2] (sentinel-collision key
+    excluded), random order, generated on the GPU; returned as int32 tensors
+    holding the uint32 bit patterns."""
+    import torch
+    from mvgpubtree_amd import workloads as wl
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    want = n + extra
+    u = torch.empty(0, dtype=torch.int64, device=dev)
+    while u.numel() < want:
+        d = torch.randint(1, 0xFFFFFFFF, (int((want - u.numel()) * 1.03) + 4096,), generator=g,
+                          device=dev, dtype=torch.int64)
+        u = torch.unique(torch.cat([u, d]))
+        u = u[u != wl.SENTINEL_COLLISION]
+    u = u[torch.randperm(u.numel(), generator=g, device=dev)[:want]]
+    u32 = torch.where(u >= 2**31, u - 2**32, u).to(torch.int32)
+    return u32[:n].contiguous(), u32[n:].contiguous()
+
+
+def run_config5(args):
+    """Served form (config 5): 500M-key tree resident in HBM; open-loop Poisson
+    arrivals at fixed QPS steps; request sizes log-uniform in [1, 16384] ops,
+    80% find / 10% range (~32 keys) / 10% insert, one snapshot per request;
+    continuous batching vs the plain per-request launch path."""
+    import torch
+
+    import mvgpubtree_amd as mv
+    from mvgpubtree_amd import serving
+    from mvgpubtree_amd import workloads as wl
+
+    dev = torch.device("cuda:0")
+    n = args.served_keys
+    levels = [float(q) for q in args.qps.split(",")]
+    n_req = args.requests
+    t0 = time.perf_counter()
+    init, fresh = gpu_unique_keys(n, (len(levels) + 1) * n_req * 2000, seed=5, dev=dev)
+    vals = init ^ torch.tensor(wl.VALUE_XOR - 2**32, dtype=torch.int32, device=dev)
+    pool = int(n / 6) + (8 << 20)
+    tree = mv.GpuVersionedBTree(pool_nodes=pool, retire_capacity=1 << 24)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    tree.insert(init, vals)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - tb
+    log(f"served: built {n} keys in {build_s:.2f}s (setup {time.perf_counter() - t0:.1f}s)")
+    rng = np.random.default_rng(55)
+    sample = np.sort(init[torch.randint(0, n, (4_000_000,), device=dev)].cpu().numpy().view(np.uint32))
+    fresh_h = fresh.cpu().numpy().view(np.uint32)
+    del init, vals, fresh
+    fi = 0
+
+    def requests(qps, count):
+        nonlocal fi
+        r = serving.make_requests(rng, count, qps, sample, fresh_h[fi:], range_capacity=128, n_live=n)
+        fi += sum(b.insert_keys.size for b in r.batches)
+        return r
+
+    # warmup: both paths, untimed
+    w = requests(1e5, 64)
+    serving.ContinuousBatchingServer(tree, w, dev).run()
+    srv = serving.PerRequestServer(tree, w, dev)
+    for i in range(16):
+        srv.runner.run_one(tree, i)
+    torch.cuda.synchronize()
+    tree.reclaim()
+    out = []
+    for q in levels:
+        reqs = requests(q, n_req)
+        s_sum, _ = serving.ContinuousBatchingServer(tree, reqs, dev, max_step_ops=args.step_ops).run()
+        p_sum, _ = serving.PerRequestServer(tree, reqs.__class__(
+            requests(q, n_req).batches, reqs.arrivals, reqs.range_capacity), dev).run()
+        tree.reclaim()
+        out.append({"qps": q, "served": s_sum, "plain": p_sum})
+        log(f"qps {q:.0f}: served {s_sum['Mops']} M ops/s p50 {s_sum['p50_ms']} ms p99 {s_sum['p99_ms']} ms"
+            f" | plain {p_sum['Mops']} M ops/s p50 {p_sum['p50_ms']} ms p99 {p_sum['p99_ms']} ms")
+    flags = tree.error_flags()
+    mem = tree.memory_stats()
+    tree.close()
+    if flags:
+        raise RuntimeError(f"device error flags 0x{flags:x}")
+    return {"keys": n, "build_Minserts_per_s": round(n / build_s / 1e6, 1), "requests_per_level": n_req,
+            "levels": out, "pool_live_GB": round(mem.live_bytes / 1e9, 2),
+            "pool_peak_GB": round(mem.peak_bytes / 1e9, 2)}
+
+
 def load_traffic(kernel: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, copy-kernel calibrated), when a summary has been recorded."""
@@ -257,6 +425,17 @@ def main():
     ap.add_argument("--ops", type=int, default=1_000_000)
     ap.add_argument("--node-width", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 4, 5),
+                    help="2: headline mixed insert/range batches; 4: batched form vs plain; "
+                         "5: served form vs plain per-request path")
+    ap.add_argument("--served-keys", type=int, default=500_000_000, help="config 5: tree size")
+    ap.add_argument("--qps", default="2000,8000,32000,128000", help="config 5: QPS steps")
+    ap.add_argument("--requests", type=int, default=2000, help="config 5: requests per QPS step")
+    ap.add_argument("--step-ops", type=int, default=1 << 20, help="config 5: max ops per served step")
+    ap.add_argument("--batches", type=int, default=4096, help="config 4: number of batches")
+    ap.add_argument("--tune-batches", type=int, default=48, help="config 4: batches per tuning group")
+    ap.add_argument("--widths", default="16,32,64", help="config 4: node widths to tune")
+    ap.add_argument("--waves", default="8,16,24,32", help="config 4: waves/CU to tune")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -267,6 +446,12 @@ def main():
         print("bench.py measures one GPU: the tree is a single-device structure here "
               "(multi-GPU key-range sharding is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.config == 4:
+        print(json.dumps({"config4": run_config4(args)}), flush=True)
+        return
+    if args.config == 5:
+        print(json.dumps({"config5": run_config5(args)}), flush=True)
+        return
     result = run_config2(args)
     print(json.dumps(result), flush=True)
 

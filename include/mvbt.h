@@ -185,9 +185,11 @@ typedef struct mvbt_op {
 
 /* No single reference counterpart: it generalises the reference's mixed kernels
  * concurrent_insert_range / concurrent_find_erase (gpu_versioned_blink_tree.hpp:218-294,
- * btree_kernels.hpp:31-306) to any mix, deterministically. Runs num_batches batches
- * (device array batch_offsets[num_batches+1], offsets[0] = 0, last = num_ops) in one
- * persistent work-queue launch. Batch b reads (find, range) at snapshot base + b, i.e.
+ * btree_kernels.hpp:31-306) to any mix, deterministically. Runs num_batches batches in
+ * one persistent work-queue launch. batch_offsets (device, num_batches + 1 entries) are
+ * absolute indices into ops: batch b is ops[offsets[b] .. offsets[b+1]), results go to
+ * the same indices, and offsets[num_batches] - offsets[0] must equal num_ops (so a
+ * launch can cover any slice of a larger pre-staged op array). Batch b reads (find, range) at snapshot base + b, i.e.
  * it sees exactly the writes of the batches before it; identical to calling
  * take_snapshot + find/range + insert/erase batch by batch. results[i]: find -> value
  * or MVBT_NOT_FOUND; range -> true count (at most L pairs written); insert -> 1 new
@@ -199,6 +201,11 @@ mvbt_status mvbt_run_batches(mvbt_tree* tree, const mvbt_op* ops, uint64_t num_o
                              const uint64_t* batch_offsets, uint32_t num_batches,
                              uint32_t* results, mvbt_pair* pairs, uint64_t pair_slots, uint32_t L,
                              uint32_t* snapshot_base, void* stream, uint32_t flags);
+
+/* Stream-ordered memory accounting: computes the mvbt_memstats of the point in
+ * `stream` where it is enqueued and stores it to `out` (device or pinned host
+ * memory) without synchronising the host. No reference counterpart. */
+mvbt_status mvbt_memory_stats_async(mvbt_tree* tree, mvbt_memstats* out, void* stream);
 
 /* Worker occupancy of the batched form in waves per CU (autotuner knob). */
 mvbt_status mvbt_set_worker_waves(mvbt_tree* tree, uint32_t waves_per_cu);

@@ -49,6 +49,7 @@ EXPORTED_SYMBOLS = (
     "mvbt_range_query", "mvbt_take_snapshot", "mvbt_release_snapshot", "mvbt_reclaim",
     "mvbt_memory_stats", "mvbt_error_flags", "mvbt_clear_error_flags", "mvbt_validate",
     "mvbt_export", "mvbt_node_width", "mvbt_kind_of", "mvbt_run_batches", "mvbt_set_worker_waves",
+    "mvbt_memory_stats_async",
 )
 
 
@@ -132,6 +133,7 @@ def load_library():
         "mvbt_kind_of": ([vp, u32p], i32),
         "mvbt_run_batches": ([vp, vp, u64, vp, u32, vp, vp, u64, u32, vp, vp, u32], i32),
         "mvbt_set_worker_waves": ([vp, u32], i32),
+        "mvbt_memory_stats_async": ([vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -162,6 +164,16 @@ def _ptr(t, elem_bytes: int, name: str):
         raise ValueError(f"{name} must live in device memory")
     if t.element_size() != elem_bytes or not t.is_contiguous():
         raise ValueError(f"{name} must be contiguous with {elem_bytes}-byte elements")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _ptr_any(t, elem_bytes: int, name: str):
+    """Like _ptr, but also accepts pinned host tensors."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or t.element_size() != elem_bytes or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous tensor with {elem_bytes}-byte elements")
+    if not (t.is_cuda or t.is_pinned()):
+        raise ValueError(f"{name} must be device or pinned host memory")
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -252,6 +264,12 @@ class _TreeBase:
         _check(load_library().mvbt_memory_stats(self._h, ctypes.byref(s)), "memory_stats")
         return MemoryStats(**{f: getattr(s, f) for f, _ in _MemStats._fields_})
 
+    def memory_stats_async(self, out, stream=None):
+        """Stream-ordered stats into `out` (8 uint64 words in device or pinned
+        host memory: a torch int64 tensor of 8 elements)."""
+        _check(load_library().mvbt_memory_stats_async(self._h, _ptr_any(out, 8, "out"), _stream(stream)),
+               "memory_stats_async")
+
     def error_flags(self) -> int:
         f = ctypes.c_uint32()
         _check(load_library().mvbt_error_flags(self._h, ctypes.byref(f)), "error_flags")
@@ -300,12 +318,14 @@ class GpuVersionedBTree(_TreeBase):
 
     def run_batches(self, ops, batch_offsets, results=None, pairs=None, pair_slots: int = 0,
                     average_range_length: int = 0, snapshot_base=None, stream=None,
-                    use_graph: bool = False):
+                    use_graph: bool = False, num_ops: int | None = None):
         """Batched form: `ops` is an int32 device tensor of shape (n, 4) holding
         {type, key, arg, slot} (OP_* types); `batch_offsets` an int64 device
-        tensor of num_batches + 1 op offsets. Returns the results tensor."""
+        tensor of num_batches + 1 absolute op indices into `ops`; results go to
+        the same indices. `num_ops` = offsets[-1] - offsets[0] (default: all of
+        `ops`). Returns the results tensor."""
         torch = _torch()
-        n = ops.shape[0]
+        n = ops.shape[0] if num_ops is None else num_ops
         if results is None:
             results = torch.empty(n, dtype=torch.int32, device=ops.device)
         nb = batch_offsets.numel() - 1

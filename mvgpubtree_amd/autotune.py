@@ -1,0 +1,97 @@
+"""Autotuner for the batched form.
+
+Sweeps node width (16, 32, 64 pairs; the tile width equals the node width,
+one pair per lane) and worker occupancy (waves per CU), times the batched
+form on a sample of the target workload against a tree of the target size,
+and returns the fastest configuration. Results can be cached in a JSON file
+keyed by device name and workload shape.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from dataclasses import asdict, dataclass
+
+import numpy as np
+
+from . import drivers
+
+
+@dataclass
+class TuneResult:
+    node_width: int
+    waves_per_cu: int
+    ms: float
+    mops: float
+
+
+def _device_name() -> str:
+    import torch
+    return torch.cuda.get_device_name(0)
+
+
+def tune_batched(initial_keys, initial_values, groups, L: int, dev,
+                 node_widths=(16, 32, 64), waves=(8, 16, 24, 32), cache: str | None = None,
+                 log=None):
+    """Returns (best TuneResult, all results).
+
+    `groups` holds 2 * len(waves) disjoint lists of batches cut from one
+    workload (fresh insert keys and distinct erase keys throughout): group 2i
+    warms waves[i] up (graph build for that grid, code load), group 2i + 1 is
+    timed at waves[i]. Every node width gets its own tree built from the full
+    initial key set, so each configuration starts from the same tree."""
+    import torch
+
+    import mvgpubtree_amd as mv
+
+    if len(groups) != 2 * len(waves):
+        raise ValueError("need 2 * len(waves) batch groups")
+    key = f"{_device_name()}|{initial_keys.size}|{[len(g) for g in groups]}|{L}"
+    if cache and os.path.exists(cache):
+        try:
+            data = json.load(open(cache))
+            if key in data:
+                best = TuneResult(**data[key]["best"])
+                return best, [TuneResult(**r) for r in data[key]["all"]]
+        except (OSError, ValueError, TypeError):
+            pass
+    runners = [drivers.BatchedRunner(g, L, dev, seed=i) for i, g in enumerate(groups)]
+    total_ops = sum(r.nops for r in runners)
+    dk = drivers._dev(initial_keys, dev)
+    dv = drivers._dev(initial_values, dev)
+    results = []
+    for nw in node_widths:
+        pool = int(initial_keys.size / 4 * (16 / nw) + 2 * total_ops) + (1 << 20)
+        tree = mv.GpuVersionedBTree(pool_nodes=pool, node_width=nw, retire_capacity=1 << 24)
+        tree.insert(dk, dv)
+        for i, wv in enumerate(waves):
+            tree.set_worker_waves(wv)
+            runners[2 * i].run(tree, use_graph=True)
+            tree.reclaim()
+            runner = runners[2 * i + 1]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            runner.run(tree, use_graph=True)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            tree.reclaim()
+            r = TuneResult(nw, wv, ms, runner.nops / ms / 1e3)
+            results.append(r)
+            if log:
+                log(f"autotune nw={nw} waves/CU={wv}: {ms:.2f} ms ({r.mops:.1f} M ops/s)")
+        if tree.error_flags():
+            raise RuntimeError(f"autotune: device error flags 0x{tree.error_flags():x}")
+        tree.close()
+    best = max(results, key=lambda r: r.mops)
+    if cache:
+        data = {}
+        if os.path.exists(cache):
+            try:
+                data = json.load(open(cache))
+            except (OSError, ValueError):
+                data = {}
+        data[key] = {"best": asdict(best), "all": [asdict(r) for r in results]}
+        with open(cache, "w") as f:
+            json.dump(data, f, indent=1)
+    return best, results

@@ -6,8 +6,8 @@
 // writes (insert, erase) are stamped clock0 + b + 1. The launch reserves the
 // nb snapshot ids by advancing the tree clock by nb.
 //
-// Scheduling: tiles take chunks of W consecutive operations from one queue in
-// order. An operation of batch b may run once every write of batches < b has
+// Scheduling: tiles claim operations one at a time from one queue, in order
+// (see the worker). An operation of batch b may run once every write of batches < b has
 // completed (a device frontier F = last batch whose writes are all done). A
 // write of batch b must also wait for that, because leaf version chains are
 // stamp-ordered: a late write of batch b-1 must never follow a copy made by
@@ -36,8 +36,8 @@ struct op_record {
 struct alignas(128) batch_job {
   // per-call parameters (written by batch_setup_kernel)
   const op_record* ops;
-  unsigned long long nops;
-  const unsigned long long* offsets;  // nb + 1 op offsets, offsets[0] = 0
+  unsigned long long nops;  // one past the last op index (= offsets[nb])
+  const unsigned long long* offsets;  // nb + 1 absolute op indices into ops
   uint32_t nb;
   uint32_t L;
   uint32_t* results;     // one word per op
@@ -52,7 +52,7 @@ struct alignas(128) batch_job {
   int32_t* advance;
   unsigned long long pad1[3];
   // hot runtime words, one line each
-  unsigned long long next_op;
+  unsigned long long next_op;  // next unclaimed operation index
   unsigned long long pad2[15];
   int32_t frontier;
   int32_t pad3[31];
@@ -74,7 +74,6 @@ struct batch_params {
 __global__ void batch_setup_kernel(batch_job* job, tree_state* st, batch_params p) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   job->ops = p.ops;
-  job->nops = p.nops;
   job->offsets = p.offsets;
   job->nb = p.nb;
   job->L = p.L;
@@ -83,7 +82,11 @@ __global__ void batch_setup_kernel(batch_job* job, tree_state* st, batch_params 
   job->pair_slots = p.pair_slots;
   job->base_out = p.base_out;
   job->clock0 = __hip_atomic_fetch_add(&st->clock, p.nb, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  job->next_op = 0;
+  // Operation indices are absolute: batch b covers ops [offsets[b], offsets[b+1]),
+  // results go to the same indices. The caller checked offsets[nb] - offsets[0]
+  // equals the op count it passed.
+  job->next_op = p.offsets[0];
+  job->nops = p.offsets[p.nb];
   job->frontier = -1;
 }
 
@@ -134,7 +137,7 @@ __global__ void batch_finish_kernel(batch_job* job) {
 }
 
 MVBT_DEV uint32_t batch_of(const unsigned long long* offsets, uint32_t nb, unsigned long long i) {
-  uint32_t lo = 0, hi = nb;  // offsets[lo] <= i < offsets[hi]
+  uint32_t lo = 0, hi = nb;  // offsets[lo] <= i < offsets[hi] (i in [offsets[0], offsets[nb]))
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
     if (offsets[mid] <= i) lo = mid;
@@ -143,7 +146,15 @@ MVBT_DEV uint32_t batch_of(const unsigned long long* offsets, uint32_t nb, unsig
   return lo;
 }
 
-// 4. The persistent worker.
+// 4. The persistent worker. Each tile holds one operation at a time, so a
+// batch spreads over as many tiles as it has operations and a batch's writes
+// finish after about one write latency. The idle tiles of a wave claim the
+// next operations of the single queue together, with one atomic (ranked by
+// tile order). Claims are in operation order, so the lowest unexecuted
+// operation is always held by a running tile and runnable: the queue drains
+// without deadlock however many of the launch's blocks are resident (a block
+// claims only while it runs). Per-wave home queues would lose this: ops of a
+// queue whose waves are not resident could block every resident tile.
 template <int NW>
 __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_job* job) {
   const tree_view<NW, true> tv(tp);
@@ -154,29 +165,35 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
   uint32_t* results = job->results;
   uint64_t* pairs = job->pairs;
   const unsigned long long pair_slots = job->pair_slots;
+  const uint32_t lane = __lane_id();
+  const uint32_t my_tile = lane / NW;  // tile index within the wave
   write_ctx ctx = tv.begin_writes(t);
-  int32_t seen = -1;  // last frontier value observed (tile-uniform)
-  unsigned long long base = 0;
-  uint32_t len = 0, pos = 0;
-  op_record mine{};
-  uint32_t my_batch = 0, my_result = 0;
-  uint32_t waits = 0;  // consecutive passes without progress
+  int32_t seen = -1;      // last frontier value observed (tile-uniform)
+  bool have = false;      // tile holds an operation (tile-uniform)
+  unsigned long long idx = 0;
+  op_record op{};
+  uint32_t b = 0;
+  uint32_t waits = 0;     // consecutive passes without progress
   for (;;) {
-    if (pos == len) {
-      if (len && t.rank() < len) results[base + t.rank()] = my_result;
-      unsigned long long b0 = 0;
-      if (t.leader()) b0 = atomic_add64(&job->next_op, (unsigned long long)NW);
-      base = t.shfl64(b0, 0);
-      if (base >= nops) break;
-      len = (uint32_t)((nops - base) < (unsigned long long)NW ? (nops - base) : NW);
-      pos = 0;
-      if (t.rank() < len) {
-        mine = ops[base + t.rank()];
-        my_batch = batch_of(job->offsets, nb, base + t.rank());
+    // Idle tiles of this wave claim operations together.
+    const uint64_t idle = __builtin_amdgcn_ballot_w64(!have && t.leader());
+    if (idle) {
+      const uint32_t n_idle = (uint32_t)__builtin_popcountll(idle);
+      const uint32_t first = (uint32_t)__builtin_ctzll(idle);
+      unsigned long long base = 0;
+      if (lane == first) base = atomic_add64(&job->next_op, (unsigned long long)n_idle);
+      base = ((unsigned long long)__builtin_amdgcn_readlane((int)(uint32_t)(base >> 32), (int)first) << 32) |
+             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, (int)first);
+      if (!have) {
+        // rank of this tile among the idle tiles (tile leaders are lanes my_tile*NW)
+        const uint64_t below = (my_tile == 0) ? 0ull : (idle & ((1ull << (my_tile * NW)) - 1));
+        idx = base + (unsigned long long)__builtin_popcountll(below);
+        if (idx >= nops) break;  // queue drained
+        op = ops[idx];
+        b = batch_of(job->offsets, nb, idx);
+        have = true;
       }
-      my_result = kEmpty;
     }
-    const uint32_t b = t.shfl(my_batch, pos);
     if ((int32_t)b - 1 > seen) {
       int32_t f = 0;
       if (t.leader()) f = __hip_atomic_load(&job->frontier, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -186,27 +203,24 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
           tv.flag(kErrCorrupt, t);
           break;
         }
-        __builtin_amdgcn_s_sleep(4);
+        __builtin_amdgcn_s_sleep(2);
         continue;
       }
     }
     waits = 0;
-    const uint32_t ty = t.shfl(mine.type, pos), key = t.shfl(mine.key, pos);
-    const uint32_t arg = t.shfl(mine.arg, pos);
     uint32_t r;
-    if (ty == kOpFind) {
-      r = tv.template find<true>(key, clock0 + b, t);
-    } else if (ty == kOpRange) {
-      const uint32_t slot = t.shfl(mine.slot, pos);
-      uint64_t* out = (pairs && slot < pair_slots) ? pairs + (size_t)slot * L : nullptr;
-      if (pairs && slot >= pair_slots) tv.flag(kErrCorrupt, t);
-      r = tv.template range<true>(key, arg, clock0 + b, out, L, t);
+    if (op.type == kOpFind) {
+      r = tv.template find<true>(op.key, clock0 + b, t);
+    } else if (op.type == kOpRange) {
+      uint64_t* out = (pairs && op.slot < pair_slots) ? pairs + (size_t)op.slot * L : nullptr;
+      if (pairs && op.slot >= pair_slots) tv.flag(kErrCorrupt, t);
+      r = tv.template range<true>(op.key, op.arg, clock0 + b, out, L, t);
       if (r > L) tv.flag(kErrRangeOverflow, t);
     } else {
       ctx.cur = clock0 + b + 1;
-      ctx.want = len - pos;
-      r = (ty == kOpInsert) ? tv.template write<0>(key, arg, ctx, t)
-                            : tv.template write<1>(key, 0u, ctx, t);
+      ctx.want = NW;  // refills grow with demand (write_ctx::grow)
+      r = (op.type == kOpInsert) ? tv.template write<0>(op.key, op.arg, ctx, t)
+                                 : tv.template write<1>(op.key, 0u, ctx, t);
       order_release();  // this write's stores complete before it is counted
       if (t.leader()) {
         uint32_t prev = atomic_add32(job->done + b, 1u);
@@ -214,8 +228,8 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
           __hip_atomic_store(&job->frontier, job->advance[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    if (t.rank() == pos) my_result = r;
-    ++pos;
+    if (t.leader()) results[idx] = r;
+    have = false;
   }
   tv.finish(ctx, t);
 }

@@ -696,6 +696,7 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
   if (!t->versioned) return fail(MVBT_ERR_UNSUPPORTED, "the batched form needs a versioned tree");
   if (num_ops == 0 || num_batches == 0) return MVBT_OK;
   if (!ops || !batch_offsets || !results) return fail(MVBT_ERR_INVALID_ARG, "null ops, offsets or results");
+  if (num_batches > (1u << 30)) return fail(MVBT_ERR_INVALID_ARG, "too many batches");
   if (pairs && L == 0) return fail(MVBT_ERR_INVALID_ARG, "L must be > 0 when pairs is given");
   device_guard g(t->device);
   std::lock_guard<std::mutex> lk(t->mu);
@@ -739,6 +740,17 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
   }
   MVBT_HIP(hipGetLastError());
   return after_write(t, s);
+}
+
+mvbt_status mvbt_memory_stats_async(mvbt_tree* t, mvbt_memstats* out, void* stream) {
+  static_assert(sizeof(mvbt_memstats) == 8 * sizeof(unsigned long long), "memstats layout");
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!out) return fail(MVBT_ERR_INVALID_ARG, "null out");
+  device_guard g(t->device);
+  hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, t->st,
+                     (unsigned long long)t->nw * 8, reinterpret_cast<unsigned long long*>(out));
+  MVBT_HIP(hipGetLastError());
+  return MVBT_OK;
 }
 
 mvbt_status mvbt_set_worker_waves(mvbt_tree* t, uint32_t waves_per_cu) {

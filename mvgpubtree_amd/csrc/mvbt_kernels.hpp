@@ -153,6 +153,24 @@ __global__ void reclaim_prep_kernel(tree_state* st, unsigned long long end_pos) 
   st->reclaimed += count;
 }
 
+// Stream-ordered memory statistics (same fields as mvbt_memstats).
+__global__ void stats_kernel(const tree_state* st, unsigned long long node_bytes,
+                             unsigned long long* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long on_stack = (st->free_top > 0 ? (unsigned long long)st->free_top : 0ull) +
+                                st->spill_count;
+  unsigned long long bump = st->bump < st->capacity ? st->bump : st->capacity;
+  unsigned long long live = bump - on_stack;
+  out[0] = node_bytes;
+  out[1] = st->capacity;
+  out[2] = live;
+  out[3] = st->peak_live > live ? st->peak_live : live;
+  out[4] = st->capacity - bump + on_stack;
+  out[5] = st->retire_tail - st->retire_head;
+  out[6] = st->reclaimed;
+  out[7] = st->retire_tail;
+}
+
 // Reclamation, phase 2: grid-stride copy of the freed ring entries and the
 // parked spill slots onto the free stack.
 __global__ void reclaim_copy_kernel(tree_state* st, const uint32_t* ring, const uint32_t* spill,

@@ -83,7 +83,8 @@ struct write_ctx {
   uint32_t mag_n;  // tile-uniform
   uint32_t ret_n;  // tile-uniform
   uint32_t cur;    // write stamp (tile-uniform)
-  uint32_t want;   // operations this tile still has queued (sizes refills)
+  uint32_t want;   // writes this tile still has queued (caps refills)
+  uint32_t grow;   // next refill size: starts at 2, doubles up to W
 };
 
 // Device pointers of one tree, independent of the node width (passed to
@@ -132,6 +133,7 @@ struct tree_view : tree_ptrs {
     c.ret_n = 0;
     c.cur = refresh_stamp(t);
     c.want = NW;
+    c.grow = 2;
     return c;
   }
   // Write stamp, read once per chunk of operations instead of once per op.
@@ -145,11 +147,14 @@ struct tree_view : tree_ptrs {
   // and slab_alloc.hpp:131-205). Refills take W recycled indices from the free
   // stack, the rest from the never-used frontier. Returns kNone when the pool
   // is exhausted.
-  // Refill size: a versioned write copies its leaf once per epoch, so it needs
-  // about one slot per queued operation (+2 for splits); a plain write needs a
-  // slot only to split (2 for the root).
-  MVBT_DEV uint32_t refill_size(const write_ctx& c) const {
+  // Refill size: grows geometrically (2, 4, 8, ... W) with the tile's actual
+  // demand, so a tile never holds more than about twice what it has used, and
+  // is capped by the writes it still has queued (+2 for splits). A plain tree
+  // writes in place and allocates only to split (2 for the root).
+  MVBT_DEV uint32_t refill_size(write_ctx& c) const {
     uint32_t k = VER ? c.want + 2 : 2;
+    if (k > c.grow) k = c.grow;
+    if (c.grow < (uint32_t)NW) c.grow *= 2;
     return k < (uint32_t)NW ? k : (uint32_t)NW;
   }
   MVBT_DEV void refill(write_ctx& c, const tile_t& t) const {

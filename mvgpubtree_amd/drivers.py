@@ -49,8 +49,18 @@ class PerBatchRunner:
         """Runs every batch; with collect=True returns per-batch host copies of
         (find values, range counts, range pairs) for parity checks."""
         out = []
-        for i, b in enumerate(self.batches):
-            s = tree.take_snapshot(stream)
+        for i in range(len(self.batches)):
+            r = self.run_one(tree, i, stream, collect)
+            if collect:
+                out.append(r)
+        return out
+
+    def run_one(self, tree, i: int, stream=None, collect: bool = False):
+        """Snapshot, reads at it, writes, release (and a periodic reclaim)."""
+        b = self.batches[i]
+        res = None
+        s = tree.take_snapshot(stream)
+        if True:
             if b["find"] is not None:
                 n = b["find"].numel()
                 tree.find(b["find"], self.find_out[:n], timestamp=s, stream=stream)
@@ -67,7 +77,7 @@ class PerBatchRunner:
                     if b["lo"] is not None else np.empty(0, np.uint32)
                 p = self.pairs[:b["lo"].numel() * self.L].cpu().numpy().view(np.uint64).copy() \
                     if b["lo"] is not None else np.empty(0, np.uint64)
-                out.append((f, c, p))
+                res = (f, c, p)
             if b["ik"] is not None:
                 tree.insert(b["ik"], b["iv"], stream=stream)
             if b["ek"] is not None:
@@ -75,7 +85,7 @@ class PerBatchRunner:
             tree.release_snapshot(s)
             if self.reclaim_every and (i + 1) % self.reclaim_every == 0:
                 tree.reclaim(stream)
-        return out
+        return res
 
 
 class BatchedRunner:
