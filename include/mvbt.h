@@ -210,6 +210,28 @@ mvbt_status mvbt_memory_stats_async(mvbt_tree* tree, mvbt_memstats* out, void* s
 /* Worker occupancy of the batched form in waves per CU (autotuner knob). */
 mvbt_status mvbt_set_worker_waves(mvbt_tree* tree, uint32_t waves_per_cu);
 
+/* ---- Device-side cooperative API ------------------------------------------ */
+
+/* Device pointers of a tree for user kernels (include/mvbt_device.hpp turns it
+ * into GpuBTree::device::*_tree with cooperative_find/insert/erase/range_query
+ * and take_snapshot, the reference's device API, gpu_versioned_blink_tree.hpp:304-559).
+ * After it has been called the handle treats every write call as possibly
+ * following user-kernel writes. */
+typedef struct mvbt_device_handle {
+  void* nodes;
+  void* state;
+  void* free_stack;
+  void* spill;
+  void* retire_ring;
+  uint64_t retire_capacity;
+  uint32_t root;
+  uint32_t capacity;
+  uint32_t node_width;
+  uint32_t kind;
+} mvbt_device_handle;
+
+mvbt_status mvbt_device_view(mvbt_tree* tree, mvbt_device_handle* out);
+
 /* Tree parameters. */
 mvbt_status mvbt_node_width(mvbt_tree* tree, uint32_t* node_width);
 mvbt_status mvbt_kind_of(mvbt_tree* tree, uint32_t* kind);

@@ -49,7 +49,7 @@ EXPORTED_SYMBOLS = (
     "mvbt_range_query", "mvbt_take_snapshot", "mvbt_release_snapshot", "mvbt_reclaim",
     "mvbt_memory_stats", "mvbt_error_flags", "mvbt_clear_error_flags", "mvbt_validate",
     "mvbt_export", "mvbt_node_width", "mvbt_kind_of", "mvbt_run_batches", "mvbt_set_worker_waves",
-    "mvbt_memory_stats_async",
+    "mvbt_memory_stats_async", "mvbt_device_view",
 )
 
 
@@ -134,6 +134,7 @@ def load_library():
         "mvbt_run_batches": ([vp, vp, u64, vp, u32, vp, vp, u64, u32, vp, vp, u32], i32),
         "mvbt_set_worker_waves": ([vp, u32], i32),
         "mvbt_memory_stats_async": ([vp, vp, vp], i32),
+        "mvbt_device_view": ([vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -3,6 +3,7 @@
 ``build()`` compiles every time it is called:
   * ``mvgpubtree_amd/libmvbt.so``   - kernels + C ABI (include/mvbt.h)
   * ``mvgpubtree_amd/capi_example`` - C++ program using include/gpu_btree.h
+  * ``mvgpubtree_amd/device_api_example`` - user kernels on include/mvbt_device.hpp
 """
 from __future__ import annotations
 
@@ -57,9 +58,20 @@ def build_example(verbose: bool = False) -> str:
     return EXAMPLE
 
 
+def build_device_example(verbose: bool = False) -> str:
+    """User-kernel program over include/mvbt_device.hpp (device code: gfx950)."""
+    src = os.path.join(CSRC, "device_api_example.hip")
+    out = os.path.join(PKG, "device_api_example")
+    cmd = [HIPCC, *[f for f in HIP_FLAGS if f != "-fPIC"], f"-I{os.path.join(ROOT, 'include')}",
+           "-o", out, src, f"-L{PKG}", "-lmvbt", f"-Wl,-rpath,{PKG}"]
+    _run(cmd, verbose)
+    return out
+
+
 def build(verbose: bool = False) -> None:
     build_lib(verbose)
     build_example(verbose)
+    build_device_example(verbose)
 
 
 if __name__ == "__main__":

@@ -54,9 +54,12 @@ struct alignas(128) batch_job {
   // hot runtime words, one line each
   unsigned long long next_op;  // next unclaimed operation index
   unsigned long long pad2[15];
-  int32_t frontier;
-  int32_t pad3[31];
+  // kFrontierCopies replicas of the frontier, one line each: waiting waves
+  // poll the replica of their wave id, so polling spreads over L2 channels.
+  // Replicas only grow (atomic max), so no replica ever moves backwards.
+  int32_t frontier[8 * 32];
 };
+constexpr uint32_t kFrontierCopies = 8;
 
 struct batch_params {
   const op_record* ops;
@@ -87,7 +90,7 @@ __global__ void batch_setup_kernel(batch_job* job, tree_state* st, batch_params 
   // equals the op count it passed.
   job->next_op = p.offsets[0];
   job->nops = p.offsets[p.nb];
-  job->frontier = -1;
+  for (uint32_t c = 0; c < kFrontierCopies; ++c) job->frontier[c * 32] = -1;
 }
 
 // 2. Writes per batch (one block per batch, grid-stride over batches).
@@ -127,7 +130,7 @@ __global__ void batch_frontier_kernel(batch_job* job) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     uint32_t j = 0;
     while (j < nb && job->nwrites[j] == 0) ++j;
-    job->frontier = (int32_t)j - 1;
+    for (uint32_t c = 0; c < kFrontierCopies; ++c) job->frontier[c * 32] = (int32_t)j - 1;
   }
 }
 
@@ -167,6 +170,9 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
   const unsigned long long pair_slots = job->pair_slots;
   const uint32_t lane = __lane_id();
   const uint32_t my_tile = lane / NW;  // tile index within the wave
+  const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  int32_t* const my_frontier = job->frontier + (wave % kFrontierCopies) * 32;
+  uint32_t backoff = 1;  // s_sleep units while the whole wave waits
   write_ctx ctx = tv.begin_writes(t);
   int32_t seen = -1;      // last frontier value observed (tile-uniform)
   bool have = false;      // tile holds an operation (tile-uniform)
@@ -194,20 +200,31 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
         have = true;
       }
     }
-    if ((int32_t)b - 1 > seen) {
+    // Tiles whose batch is not yet open refresh their view of the frontier:
+    // one load per wave (the first such tile leader's), shared by readlane.
+    const bool blocked = (int32_t)b - 1 > seen;
+    const uint64_t need = __builtin_amdgcn_ballot_w64(blocked && t.leader());
+    if (need) {
+      const uint32_t poller = (uint32_t)__builtin_ctzll(need);
       int32_t f = 0;
-      if (t.leader()) f = __hip_atomic_load(&job->frontier, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      seen = (int32_t)t.shfl((uint32_t)f, 0);
-      if ((int32_t)b - 1 > seen) {
-        if (++waits > (1u << 24)) {  // ~seconds: the frontier never moved
-          tv.flag(kErrCorrupt, t);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-        continue;
+      if (lane == poller) f = __hip_atomic_load(my_frontier, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      f = __builtin_amdgcn_readlane(f, (int)poller);
+      if (f > seen) seen = f;
+    }
+    if ((int32_t)b - 1 > seen) {
+      if (++waits > (1u << 21)) {  // seconds: the frontier never moved
+        tv.flag(kErrCorrupt, t);
+        break;
       }
+      // back off only when no tile of the wave can work
+      if (__builtin_amdgcn_ballot_w64(!((int32_t)b - 1 > seen)) == 0) {
+        for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(8);
+        if (backoff < 16) backoff *= 2;
+      }
+      continue;
     }
     waits = 0;
+    backoff = 1;
     uint32_t r;
     if (op.type == kOpFind) {
       r = tv.template find<true>(op.key, clock0 + b, t);
@@ -224,8 +241,11 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
       order_release();  // this write's stores complete before it is counted
       if (t.leader()) {
         uint32_t prev = atomic_add32(job->done + b, 1u);
-        if (prev + 1 == job->nwrites[b])
-          __hip_atomic_store(&job->frontier, job->advance[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev + 1 == job->nwrites[b]) {
+          const int32_t adv = job->advance[b];
+          for (uint32_t c = 0; c < kFrontierCopies; ++c)
+            __hip_atomic_fetch_max(job->frontier + c * 32, adv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
     }
     if (t.leader()) results[idx] = r;

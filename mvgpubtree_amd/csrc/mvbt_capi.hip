@@ -69,6 +69,7 @@ struct mvbt_tree {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   hipGraphNode_t graph_setup = nullptr;
+  bool external_writers = false;  // a device view was handed out
 
   tree_ptrs ptrs() const {
     tree_ptrs v;
@@ -217,7 +218,8 @@ bool other_writers_busy(mvbt_tree* t, hipStream_t s) {
 // Before a write kernel on stream s (caller holds t->mu).
 mvbt_status before_write(mvbt_tree* t, hipStream_t s) {
   if (t->maint_ev) MVBT_HIP(hipStreamWaitEvent(s, t->maint_ev, 0));
-  if (t->spill_maybe && !other_writers_busy(t, s)) return launch_maintenance(t, s, 0);
+  if ((t->spill_maybe || t->external_writers) && !other_writers_busy(t, s))
+    return launch_maintenance(t, s, 0);
   return MVBT_OK;
 }
 
@@ -763,6 +765,25 @@ mvbt_status mvbt_set_worker_waves(mvbt_tree* t, uint32_t waves_per_cu) {
     destroy_graph(t);
     t->worker_waves = waves_per_cu;
   }
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_device_view(mvbt_tree* t, mvbt_device_handle* out) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!out) return fail(MVBT_ERR_INVALID_ARG, "null out");
+  std::lock_guard<std::mutex> lk(t->mu);
+  tree_ptrs p = t->ptrs();
+  out->nodes = p.nodes;
+  out->state = p.st;
+  out->free_stack = p.free_stack;
+  out->spill = p.spill;
+  out->retire_ring = p.retire_ring;
+  out->retire_capacity = p.retire_cap;
+  out->root = p.root_;
+  out->capacity = p.cap_;
+  out->node_width = t->nw;
+  out->kind = t->versioned ? MVBT_VERSIONED : MVBT_PLAIN;
+  t->external_writers = true;
   return MVBT_OK;
 }
 

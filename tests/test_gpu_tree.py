@@ -285,3 +285,14 @@ def test_reserved_keys_and_pool_exhaustion_are_flagged(dev):
     hits = got != MISS
     assert np.array_equal(got[hits], k[hits]) and hits.sum() == nk
     small.close()
+
+
+def test_device_api_user_kernels():
+    """User kernels on include/mvbt_device.hpp: cooperative_insert, an
+    in-kernel take_snapshot, then tiles inserting while other tiles read that
+    snapshot with cooperative_find / concurrent_cooperative_range_query."""
+    exe = os.path.join(os.path.dirname(mv.__file__), "device_api_example")
+    for n in ("1000", "200000"):
+        r = subprocess.run([exe, n], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "OK" in r.stdout
