@@ -365,9 +365,12 @@ def run_config5(args):
         fi += sum(b.insert_keys.size for b in r.batches)
         return r
 
-    # warmup: both paths, untimed
+    # warmup: both paths, untimed; the served loop keeps one stream throughout
+    stream = torch.cuda.Stream()
     w = requests(1e5, 64)
-    serving.ContinuousBatchingServer(tree, w, dev).run()
+    serving.ContinuousBatchingServer(tree, w, dev, stream=stream).run()
+    w = requests(2e3, 16)
+    serving.ContinuousBatchingServer(tree, w, dev, stream=stream).run()
     srv = serving.PerRequestServer(tree, w, dev)
     for i in range(16):
         srv.runner.run_one(tree, i)
@@ -376,12 +379,16 @@ def run_config5(args):
     out = []
     for q in levels:
         reqs = requests(q, n_req)
-        s_sum, _ = serving.ContinuousBatchingServer(tree, reqs, dev, max_step_ops=args.step_ops).run()
+        s_sum, _ = serving.ContinuousBatchingServer(tree, reqs, dev, max_step_ops=args.step_ops,
+                                                    stream=stream).run()
         p_sum, _ = serving.PerRequestServer(tree, reqs.__class__(
             requests(q, n_req).batches, reqs.arrivals, reqs.range_capacity), dev).run()
         tree.reclaim()
         out.append({"qps": q, "served": s_sum, "plain": p_sum})
         log(f"qps {q:.0f}: served {s_sum['Mops']} M ops/s p50 {s_sum['p50_ms']} ms p99 {s_sum['p99_ms']} ms"
+            f" (queue p99 {s_sum['queue_p99_ms']} ms, step gpu p50/p99 {s_sum['step_gpu_p50_ms']}/"
+            f"{s_sum['step_gpu_p99_ms']} ms, {s_sum['steps']} steps of {s_sum['step_ops_mean']} ops, "
+            f"enqueue p50/max {s_sum['enqueue_p50_ms']}/{s_sum['enqueue_max_ms']} ms, worst {s_sum['worst']})"
             f" | plain {p_sum['Mops']} M ops/s p50 {p_sum['p50_ms']} ms p99 {p_sum['p99_ms']} ms")
     flags = tree.error_flags()
     mem = tree.memory_stats()

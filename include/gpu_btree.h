@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -81,6 +82,11 @@ class tree_base {
                            concurrent ? MVBT_CONCURRENT : 0),
           "range_query");
   }
+  // Sorted bulk load of this new tree (fill 0 = default entries per node).
+  void bulk_build(const key_type* keys, const value_type* values, uint64_t num_keys, uint32_t fill = 0,
+                  hipStream_t stream = 0) {
+    check(mvbt_bulk_build(h_, keys, values, num_keys, fill, stream), "bulk_build");
+  }
   // Host structure check (every allocator); returns false on a broken invariant.
   bool validate_tree_structure(uint64_t* num_keys = nullptr, uint32_t* height = nullptr) {
     return mvbt_validate(h_, num_keys, height) == MVBT_OK;
@@ -112,7 +118,17 @@ class gpu_blink_tree : public detail::tree_base<B, MVBT_PLAIN> {
   static_assert(sizeof(Key) == 4 && sizeof(Value) == 4, "keys and values are 32-bit");
 
  public:
-  using detail::tree_base<B, MVBT_PLAIN>::tree_base;
+  using base = detail::tree_base<B, MVBT_PLAIN>;
+  using base::base;
+  // Reference bulk-build constructor (gpu_blink_tree.hpp:85-136): sorted,
+  // distinct keys only (the reference terminates otherwise; this throws).
+  // pool_nodes 0 sizes the pool for the build plus as many inserts again.
+  gpu_blink_tree(const Key* keys, const Value* values, typename base::size_type num_keys,
+                 bool sorted_input = false, hipStream_t stream = 0, uint64_t pool_nodes = 0)
+      : base(pool_nodes ? pool_nodes : std::max<uint64_t>(1ull << 22, uint64_t(num_keys) / 4 + (1ull << 20))) {
+    if (!sorted_input) throw error(MVBT_ERR_INVALID_ARG, "gpu_blink_tree bulk build: only sorted input is supported");
+    base::bulk_build(keys, values, num_keys, 0, stream);
+  }
 };
 
 // Multiversion B-link tree (reference gpu_versioned_btree<K,V,B,Alloc>).

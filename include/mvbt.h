@@ -165,6 +165,18 @@ mvbt_status mvbt_validate(mvbt_tree* tree, uint64_t* num_keys, uint32_t* height)
  * `capacity` pairs; *count receives the number of keys. Synchronous. */
 mvbt_status mvbt_export(mvbt_tree* tree, mvbt_pair* out, uint64_t capacity, uint64_t* count);
 
+/* Replaces the sorted-input bulk-build constructor gpu_blink_tree(keys, values,
+ * num_keys, sorted_input = true, stream) (gpu_blink_tree.hpp:85-136,
+ * bulk_build_kernel btree_kernels.hpp:915-1025), for both tree kinds. Loads n
+ * (key, value) pairs into a new tree (no write, snapshot or device view yet)
+ * with `fill` entries per node (0 = two thirds of the data lanes, rounded up;
+ * else 2 .. data lanes). keys must be strictly ascending and exclude the
+ * reserved keys; otherwise MVBT_ERR_INVALID_ARG and the tree is unchanged
+ * (the reference terminates on unsorted input). The result equals inserting
+ * the pairs. Synchronous. */
+mvbt_status mvbt_bulk_build(mvbt_tree* tree, const uint32_t* keys, const uint32_t* values, uint64_t n,
+                            uint32_t fill, void* stream);
+
 /* ---- Batched form: many batches of mixed operations in one launch ---------- */
 
 #define MVBT_OP_FIND 0u

@@ -49,7 +49,7 @@ EXPORTED_SYMBOLS = (
     "mvbt_range_query", "mvbt_take_snapshot", "mvbt_release_snapshot", "mvbt_reclaim",
     "mvbt_memory_stats", "mvbt_error_flags", "mvbt_clear_error_flags", "mvbt_validate",
     "mvbt_export", "mvbt_node_width", "mvbt_kind_of", "mvbt_run_batches", "mvbt_set_worker_waves",
-    "mvbt_memory_stats_async", "mvbt_device_view",
+    "mvbt_memory_stats_async", "mvbt_device_view", "mvbt_bulk_build",
 )
 
 
@@ -135,6 +135,7 @@ def load_library():
         "mvbt_set_worker_waves": ([vp, u32], i32),
         "mvbt_memory_stats_async": ([vp, vp, vp], i32),
         "mvbt_device_view": ([vp, vp], i32),
+        "mvbt_bulk_build": ([vp, vp, vp, u64, u32, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -142,6 +143,19 @@ def load_library():
         fn.restype = res
     _lib = L
     return L
+
+
+def sort_unique_pairs(keys, values):
+    """Device sort of uint32 keys (int32 tensors holding the bit patterns) with
+    their values; of repeated keys the last occurrence is kept."""
+    torch = _torch()
+    k64 = keys.to(torch.int64) & 0xFFFFFFFF
+    order = torch.sort(k64, stable=True).indices
+    ks, vs = k64[order], values[order]
+    keep = torch.ones_like(ks, dtype=torch.bool)
+    keep[:-1] = ks[1:] != ks[:-1]
+    ks, vs = ks[keep], vs[keep]
+    return torch.where(ks >= 2**31, ks - 2**32, ks).to(torch.int32).contiguous(), vs.contiguous()
 
 
 def _check(status: int, op: str):
@@ -217,6 +231,20 @@ class _TreeBase:
         return self._h
 
     # --- batch API (reference host methods) -------------------------------
+    def bulk_build(self, keys, values, fill: int = 0, sorted_input: bool = True, stream=None):
+        """Loads (key, value) pairs into this new tree, `fill` entries per node
+        (reference sorted-input constructor, gpu_blink_tree.hpp:85-136). With
+        sorted_input=False the pairs are first sorted by key on the device and
+        repeated keys keep their last value, as a sequence of inserts would."""
+        n = keys.numel()
+        if values.numel() != n:
+            raise ValueError("keys and values differ in length")
+        if not sorted_input and n:
+            keys, values = sort_unique_pairs(keys, values)
+            n = keys.numel()
+        _check(load_library().mvbt_bulk_build(self._h, _ptr(keys, 4, "keys"), _ptr(values, 4, "values"),
+                                              n, fill, _stream(stream)), "bulk_build")
+
     def insert(self, keys, values, stream=None):
         n = keys.numel()
         if values.numel() != n:

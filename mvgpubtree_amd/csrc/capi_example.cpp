@@ -159,6 +159,46 @@ void versioned_cases(uint32_t n, uint32_t L) {
   (void)hipHostFree(pairs);
 }
 
+// Reference bulk-build constructor: sorted keys 2, 4, ..., 2n; then finds of
+// every key and every odd (absent) key, and a few inserts on the loaded tree.
+void bulk_cases(uint32_t n) {
+  uint32_t* keys = pinned<uint32_t>(n);
+  uint32_t* vals = pinned<uint32_t>(n);
+  uint32_t* absent = pinned<uint32_t>(n);
+  uint32_t* out = pinned<uint32_t>(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    keys[i] = 2 * (i + 1);
+    vals[i] = keys[i] ^ 0x9E3779B9u;
+    absent[i] = keys[i] + 1;
+  }
+  GpuBTree::gpu_blink_tree<uint32_t, uint32_t, 16> tree(keys, vals, n, /*sorted_input=*/true);
+  uint64_t count = 0;
+  EXPECT(tree.validate_tree_structure(&count), "structure after bulk build");
+  EXPECT(count == n, "bulk key count %llu != %u", (unsigned long long)count, n);
+  tree.find(keys, out, n);
+  (void)hipDeviceSynchronize();
+  for (uint32_t i = 0; i < n; ++i) EXPECT(out[i] == vals[i], "bulk FindExist key %u got %u", keys[i], out[i]);
+  tree.find(absent, out, n);
+  (void)hipDeviceSynchronize();
+  for (uint32_t i = 0; i < n; ++i) EXPECT(out[i] == MVBT_NOT_FOUND, "bulk FindNotExist key %u got %u", absent[i], out[i]);
+  tree.insert(absent, absent, n);
+  tree.find(absent, out, n);
+  (void)hipDeviceSynchronize();
+  for (uint32_t i = 0; i < n; ++i) EXPECT(out[i] == absent[i], "insert after bulk key %u got %u", absent[i], out[i]);
+  EXPECT(tree.validate_tree_structure(&count) && count == 2ull * n, "structure after inserts");
+  bool threw = false;
+  try {
+    GpuBTree::gpu_blink_tree<uint32_t, uint32_t, 16> unsorted(keys, vals, n, /*sorted_input=*/false);
+  } catch (const GpuBTree::error&) {
+    threw = true;
+  }
+  EXPECT(threw, "unsorted bulk build must throw");
+  (void)hipHostFree(keys);
+  (void)hipHostFree(vals);
+  (void)hipHostFree(absent);
+  (void)hipHostFree(out);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -168,6 +208,7 @@ int main(int argc, char** argv) {
     plain_cases<GpuBTree::gpu_blink_tree<uint32_t, uint32_t, 16>>(n);
     plain_cases<GpuBTree::gpu_versioned_btree<uint32_t, uint32_t, 16>>(n);
     versioned_cases(n, L);
+    bulk_cases(n);
   } catch (const std::exception& e) {
     std::printf("FAIL exception: %s\n", e.what());
     return 2;

@@ -14,6 +14,7 @@
 
 #include "../../include/mvbt.h"
 #include "mvbt_batch.hpp"
+#include "mvbt_bulk.hpp"
 #include "mvbt_kernels.hpp"
 
 using namespace mvbt;
@@ -70,6 +71,7 @@ struct mvbt_tree {
   hipGraphExec_t graph_exec = nullptr;
   hipGraphNode_t graph_setup = nullptr;
   bool external_writers = false;  // a device view was handed out
+  bool fresh = true;              // no write, snapshot or device view yet (bulk build allowed)
 
   tree_ptrs ptrs() const {
     tree_ptrs v;
@@ -154,6 +156,17 @@ struct impl {
     MVBT_HIP(hipGetLastError());
     return MVBT_OK;
   }
+  static mvbt_status bulk(mvbt_tree* t, const uint32_t* keys, const uint32_t* vals, const bulk_plan& plan,
+                          hipStream_t s) {
+    const uint64_t tiles_per_block = kBlock / NW;
+    const uint64_t blocks = (plan.total + tiles_per_block - 1) / tiles_per_block;
+    const uint64_t cap = (uint64_t)t->num_cus * 32 * 64 / kBlock;
+    hipLaunchKernelGGL((bulk_build_kernel<NW, VER>), dim3((uint32_t)std::min(blocks, cap)), dim3(kBlock), 0,
+                       s, t->template view<NW, VER>(), keys, vals, plan);
+    MVBT_HIP(hipGetLastError());
+    return MVBT_OK;
+  }
+  static constexpr uint32_t data_lanes() { return node<NW, VER>::kData; }
 };
 
 template <int NW>
@@ -169,13 +182,15 @@ struct dispatch_table {
   mvbt_status (*insert)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, hipStream_t);
   mvbt_status (*erase)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, hipStream_t);
   void* (*worker)();
+  mvbt_status (*bulk)(mvbt_tree*, const uint32_t*, const uint32_t*, const bulk_plan&, hipStream_t);
+  uint32_t data_lanes;
 };
 
 template <int NW, bool VER>
 dispatch_table make_table() {
   using I = impl<NW, VER>;
   return dispatch_table{&I::init, &I::find, &I::range, &I::template write<0>, &I::template write<1>,
-                        VER ? &worker_fn<NW> : nullptr};
+                        VER ? &worker_fn<NW> : nullptr, &I::bulk, I::data_lanes()};
 }
 
 const dispatch_table* table_for(uint32_t nw, bool versioned) {
@@ -217,6 +232,7 @@ bool other_writers_busy(mvbt_tree* t, hipStream_t s) {
 
 // Before a write kernel on stream s (caller holds t->mu).
 mvbt_status before_write(mvbt_tree* t, hipStream_t s) {
+  t->fresh = false;
   if (t->maint_ev) MVBT_HIP(hipStreamWaitEvent(s, t->maint_ev, 0));
   if ((t->spill_maybe || t->external_writers) && !other_writers_busy(t, s))
     return launch_maintenance(t, s, 0);
@@ -583,6 +599,7 @@ mvbt_status mvbt_take_snapshot(mvbt_tree* t, void* stream, uint32_t* snapshot_id
   device_guard g(t->device);
   std::lock_guard<std::mutex> lk(t->mu);
   hipStream_t s = (hipStream_t)stream;
+  t->fresh = false;
   hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(64), 0, s, t->st, t->snap_rec);
   MVBT_HIP(hipGetLastError());
   MVBT_HIP(hipStreamSynchronize(s));
@@ -784,6 +801,48 @@ mvbt_status mvbt_device_view(mvbt_tree* t, mvbt_device_handle* out) {
   out->node_width = t->nw;
   out->kind = t->versioned ? MVBT_VERSIONED : MVBT_PLAIN;
   t->external_writers = true;
+  t->fresh = false;
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_bulk_build(mvbt_tree* t, const uint32_t* keys, const uint32_t* values, uint64_t n,
+                            uint32_t fill, void* stream) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (n == 0) return MVBT_OK;
+  if (!keys || !values) return fail(MVBT_ERR_INVALID_ARG, "null keys or values");
+  const dispatch_table* tb = table_for(t->nw, t->versioned);
+  if (fill == 0) fill = (2 * tb->data_lanes + 2) / 3;
+  if (fill < 2 || fill > tb->data_lanes)
+    return fail(MVBT_ERR_INVALID_ARG, "fill must be in [2, " + std::to_string(tb->data_lanes) + "]");
+  device_guard g(t->device);
+  std::lock_guard<std::mutex> lk(t->mu);
+  if (!t->fresh)
+    return fail(MVBT_ERR_INVALID_ARG, "bulk build needs a new tree (no write, snapshot or device view yet)");
+  MVBT_HIP(hipDeviceSynchronize());  // nothing else may touch the hand-off word meanwhile
+  bulk_plan plan;
+  if (!make_bulk_plan(n, fill, tb->data_lanes, &plan))
+    return fail(MVBT_ERR_INVALID_ARG, "too many levels for this fill");
+  if (plan.total > t->cfg.pool_nodes)
+    return fail(MVBT_ERR_INVALID_ARG, "node pool too small: the build needs " + std::to_string(plan.total) +
+                                          " nodes, the pool has " + std::to_string(t->cfg.pool_nodes));
+  hipStream_t s = (hipStream_t)stream;
+  // Validate first, so a rejected input leaves the tree untouched and new.
+  unsigned long long* bad = &t->st->scratch[0];
+  MVBT_HIP(hipMemsetAsync(bad, 0, sizeof(*bad), s));
+  const uint64_t blocks = std::min<uint64_t>((n + kBlock - 1) / kBlock, (uint64_t)t->num_cus * 8);
+  hipLaunchKernelGGL(bulk_check_kernel, dim3((uint32_t)blocks), dim3(kBlock), 0, s, keys,
+                     (unsigned long long)n, bad);
+  MVBT_HIP(hipGetLastError());
+  unsigned long long h_bad = 0;
+  MVBT_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost, s));
+  MVBT_HIP(hipStreamSynchronize(s));
+  if (h_bad)
+    return fail(MVBT_ERR_INVALID_ARG,
+                "bulk build keys must be strictly ascending and exclude the reserved keys 0 and 0xFFFFFFFF");
+  if (mvbt_status rc = before_write(t, s)) return rc;
+  if (mvbt_status rc = tb->bulk(t, keys, values, plan, s)) return rc;
+  if (mvbt_status rc = after_write(t, s)) return rc;
+  MVBT_HIP(hipStreamSynchronize(s));
   return MVBT_OK;
 }
 

@@ -85,6 +85,7 @@ struct write_ctx {
   uint32_t cur;    // write stamp (tile-uniform)
   uint32_t want;   // writes this tile still has queued (caps refills)
   uint32_t grow;   // next refill size: starts at 2, doubles up to W
+  uint32_t backoff;  // sleep rounds after the next failed lock (tile-uniform)
 };
 
 // Device pointers of one tree, independent of the node width (passed to
@@ -134,6 +135,7 @@ struct tree_view : tree_ptrs {
     c.cur = refresh_stamp(t);
     c.want = NW;
     c.grow = 2;
+    c.backoff = 1;
     return c;
   }
   // Write stamp, read once per chunk of operations instead of once per op.
@@ -151,6 +153,7 @@ struct tree_view : tree_ptrs {
   // demand, so a tile never holds more than about twice what it has used, and
   // is capped by the writes it still has queued (+2 for splits). A plain tree
   // writes in place and allocates only to split (2 for the root).
+  static constexpr long long kAmpleSlots = 1ll << 20;
   MVBT_DEV uint32_t refill_size(write_ctx& c) const {
     uint32_t k = VER ? c.want + 2 : 2;
     if (k > c.grow) k = c.grow;
@@ -182,6 +185,12 @@ struct tree_view : tree_ptrs {
     c.mag = idx;
     c.mag_n = (uint32_t)__builtin_popcountll(ok);
     if (c.mag_n < k) flag(kErrPoolExhausted, t);
+    // Refills start small so that slots parked unused in many magazines cannot
+    // exhaust a nearly full pool; once a refill shows ample headroom, the tile
+    // takes full magazines (one atomic on the shared counter per W slots).
+    const bool ample = top - (long long)k > kAmpleSlots ||
+                       (got < k && b + k + (unsigned long long)kAmpleSlots < (unsigned long long)cap_);
+    if (ample) c.grow = NW;
   }
   MVBT_DEV uint32_t alloc(write_ctx& c, const tile_t& t) const {
     if (c.mag_n == 0) refill(c, t);
@@ -229,10 +238,33 @@ struct tree_view : tree_ptrs {
   }
 
   // ---- locks (reference versioned_node.hpp:528-568; here try-lock only) ----
+  // Test-and-test-and-set: a held lock costs a waiter one coherent load, which
+  // the L2 serves beside other loads, instead of an atomic, which the L2
+  // serialises on the line the holder itself needs.
   MVBT_DEV bool try_lock(uint32_t idx, const tile_t& t) const {
+    uint32_t old = kLockBit;
+    if (t.leader()) {
+      old = ld32_coherent(lock_word(idx));
+      if (!(old & kLockBit)) old = atomic_or32(lock_word(idx), kLockBit);
+    }
+    return (t.shfl(old, 0) & kLockBit) == 0;
+  }
+  // The same, with the test taken from metadata the tile has just loaded
+  // coherently (no extra round trip on the uncontended path).
+  MVBT_DEV bool try_lock_seen(uint32_t idx, uint64_t meta, const tile_t& t) const {
+    if (hi32(meta) & kLockBit) return false;
     uint32_t old = 0;
     if (t.leader()) old = atomic_or32(lock_word(idx), kLockBit);
     return (t.shfl(old, 0) & kLockBit) == 0;
+  }
+  // After a failed lock: sleep, doubling the time per consecutive failure up
+  // to kMaxBackoff rounds (the whole wave sleeps, so the cap stays small).
+  // Without it, hundreds of tiles re-descending onto the few nodes of a small
+  // tree keep the lock holders' lines busy.
+  static constexpr uint32_t kMaxBackoff = 32;
+  MVBT_DEV static void back_off(write_ctx& c) {
+    for (uint32_t i = 0; i < c.backoff; ++i) __builtin_amdgcn_s_sleep(2);
+    if (c.backoff < kMaxBackoff) c.backoff <<= 1;
   }
   MVBT_DEV void unlock(uint32_t idx, const tile_t& t) const {
     order_release();
@@ -368,9 +400,12 @@ struct tree_view : tree_ptrs {
   // Splits the full root in place: its pairs move to two new children and the
   // root keeps {pivot0 -> left, pivot -> right} (reference split_as_root,
   // versioned_node.hpp:185-212, with the root never a leaf here).
-  MVBT_DEV uint32_t split_root(write_ctx& c, const tile_t& t) const {
+  MVBT_DEV uint32_t split_root(uint64_t seen_meta, write_ctx& c, const tile_t& t) const {
     uint32_t r = root_;
-    if (!try_lock(r, t)) return kAtRoot;
+    if (!try_lock_seen(r, seen_meta, t)) {
+      back_off(c);
+      return kAtRoot;
+    }
     node_t n;
     n.load(at(r), t);
     if (!n.is_full(t)) {
@@ -407,9 +442,12 @@ struct tree_view : tree_ptrs {
   // Splits full internal node `idx` whose parent is `parent` (reference
   // split, versioned_node.hpp:149-178; parent locked and re-checked as in
   // gpu_versioned_blink_tree.hpp:934-1009, but with try_lock and restart).
-  MVBT_DEV uint32_t split_internal(uint32_t idx, uint32_t parent, uint32_t key, write_ctx& c,
-                                   const tile_t& t) const {
-    if (!try_lock(idx, t)) return kAtParent;
+  MVBT_DEV uint32_t split_internal(uint32_t idx, uint32_t parent, uint32_t key, uint64_t seen_meta,
+                                   write_ctx& c, const tile_t& t) const {
+    if (!try_lock_seen(idx, seen_meta, t)) {
+      back_off(c);
+      return kAtParent;
+    }
     node_t n;
     n.load(at(idx), t);
     uint64_t m = n.meta(t);
@@ -419,6 +457,7 @@ struct tree_view : tree_ptrs {
     }
     if (!try_lock(parent, t)) {
       unlock(idx, t);
+      back_off(c);
       return kAtParent;
     }
     node_t pn;
@@ -479,6 +518,7 @@ struct tree_view : tree_ptrs {
     }
     if (!try_lock(parent, t)) {
       unlock(idx, t);
+      back_off(c);
       return kAtParent;
     }
     node_t pn;
@@ -542,9 +582,9 @@ struct tree_view : tree_ptrs {
       }
       if (!node_t::is_leaf(m)) {
         if (OP == 0 && n.is_full(t)) {  // proactive split on the way down
-          uint32_t s = (idx == root_) ? split_root(c, t)
+          uint32_t s = (idx == root_) ? split_root(m, c, t)
                        : (parent == kNone) ? (uint32_t)kAtRoot
-                                           : split_internal(idx, parent, key, c, t);
+                                           : split_internal(idx, parent, key, m, c, t);
           if (s == kOutOfMemory) return kWriteFailed;
           if (s == kAtParent && parent != kNone) idx = parent;
           else idx = root_;
@@ -555,8 +595,8 @@ struct tree_view : tree_ptrs {
         idx = n.child_for(key, t);
         continue;
       }
-      if (!try_lock(idx, t)) {
-        __builtin_amdgcn_s_sleep(2);
+      if (!try_lock_seen(idx, m, t)) {
+        back_off(c);
         continue;  // re-read the same leaf
       }
       n.load(at(idx), t);
@@ -589,6 +629,7 @@ struct tree_view : tree_ptrs {
       uint32_t rk = t.rank();
       if (node_t::data_lane(rk)) st_coherent(at(idx) + rk, n.p);
       unlock(idx, t);
+      c.backoff = c.backoff > 1 ? c.backoff >> 1 : 1u;
       return fresh ? kWriteNew : kWriteSame;
     }
   }

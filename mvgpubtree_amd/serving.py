@@ -82,7 +82,8 @@ def _percentiles(lat_s: np.ndarray):
 
 class ContinuousBatchingServer:
     def __init__(self, tree, reqs: RequestStream, dev, max_step_ops: int = 1 << 20,
-                 max_inflight: int = 2, slots_per_write: int = 7, use_graph: bool = True):
+                 max_inflight: int = 2, slots_per_write: int = 7, use_graph: bool = True,
+                 stream=None):
         import torch
         self.tree, self.reqs, self.dev = tree, reqs, dev
         self.max_step_ops = max_step_ops
@@ -95,7 +96,9 @@ class ContinuousBatchingServer:
         w = ((ty == wl.OP_INSERT) | (ty == wl.OP_ERASE)).astype(np.int64)
         cw = np.concatenate([[0], np.cumsum(w)])
         self.writes = cw[self.offsets[1:]] - cw[self.offsets[:-1]]
-        self.stream = torch.cuda.Stream()
+        # A long-lived server keeps one stream: the first launches on a new
+        # stream pay one-time runtime set-up, which belongs outside the loop.
+        self.stream = stream if stream is not None else torch.cuda.Stream()
         self.stats = [torch.zeros(8, dtype=torch.int64).pin_memory() for _ in range(max_inflight + 1)]
 
     def run(self):
@@ -108,13 +111,18 @@ class ContinuousBatchingServer:
         inflight = deque()
         nxt = k = 0
         steps = 0
+        admitted_at = np.zeros(R)
+        step_gpu_ms = []
+        step_ops = []
+        enqueue_ms = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         while k < R:
             now = time.perf_counter() - t0
             while inflight and inflight[0][0].query():
-                ev, a, b, res, stat = inflight.popleft()
+                ev, a, b, res, stat, ev0 = inflight.popleft()
                 t = time.perf_counter() - t0
+                step_gpu_ms.append(ev0.elapsed_time(ev))
                 done_at[a:b] = t
                 k = b
                 reserved -= res
@@ -139,7 +147,12 @@ class ContinuousBatchingServer:
                     raise RuntimeError("a request needs more node-pool slots than the pool has free")
                 if b > nxt:
                     stat = self.stats[steps % len(self.stats)]
+                    admitted_at[nxt:b] = now
+                    step_ops.append(ops)
+                    te = time.perf_counter()
                     with torch.cuda.stream(self.stream):
+                        ev0 = torch.cuda.Event(enable_timing=True)
+                        ev0.record(self.stream)
                         tree.run_batches(self.runner.ops, self.runner.offsets[nxt:b + 1],
                                          results=self.runner.results, pairs=self.runner.pairs,
                                          pair_slots=max(self.runner.slots, 1),
@@ -147,9 +160,10 @@ class ContinuousBatchingServer:
                                          use_graph=self.use_graph, num_ops=ops)
                         tree.reclaim(self.stream)
                         tree.memory_stats_async(stat, self.stream)
-                        ev = torch.cuda.Event()
+                        ev = torch.cuda.Event(enable_timing=True)
                         ev.record(self.stream)
-                    inflight.append((ev, nxt, b, res, stat))
+                    enqueue_ms.append((time.perf_counter() - te) * 1e3)
+                    inflight.append((ev, nxt, b, res, stat, ev0))
                     reserved += res
                     nxt = b
                     steps += 1
@@ -158,9 +172,20 @@ class ContinuousBatchingServer:
                 time.sleep(min(arrivals[nxt] - now, 1e-3) if arrivals[nxt] - now > 2e-4 else 0)
         total = time.perf_counter() - t0
         lat = done_at - arrivals
+        gpu = np.asarray(step_gpu_ms) if step_gpu_ms else np.zeros(1)
         return {"requests": R, "ops": self.reqs.num_ops, "steps": steps,
                 "Mops": round(self.reqs.num_ops / total / 1e6, 3), "wall_s": round(total, 4),
-                **_percentiles(lat)}, lat
+                **_percentiles(lat),
+                # where the latency goes: waiting for admission, then the step
+                "queue_p99_ms": round(float(np.percentile(admitted_at - arrivals, 99)) * 1e3, 4),
+                "step_gpu_p50_ms": round(float(np.percentile(gpu, 50)), 4),
+                "step_gpu_p99_ms": round(float(np.percentile(gpu, 99)), 4),
+                "step_ops_mean": round(float(np.mean(step_ops)) if step_ops else 0.0, 1),
+                "enqueue_p50_ms": round(float(np.percentile(enqueue_ms, 50)), 4) if enqueue_ms else 0.0,
+                "enqueue_max_ms": round(float(np.max(enqueue_ms)), 4) if enqueue_ms else 0.0,
+                "worst": [(int(i), round(float(lat[i]) * 1e3, 3),
+                           round(float(admitted_at[i] - arrivals[i]) * 1e3, 3))
+                          for i in np.argsort(lat)[-4:]]}, lat
 
 
 class PerRequestServer:
