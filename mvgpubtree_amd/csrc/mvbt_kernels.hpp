@@ -97,6 +97,7 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
   chunking<NW> c;
   if (c.first >= n) return;
   write_ctx ctx = tv.begin_writes(t);
+  ctx.exact = 1;  // ctx.want below is the chunk's exact remaining count
   for (uint64_t base = c.first; base < n; base += c.stride) {
     if (base != c.first) ctx.cur = tv.refresh_stamp(t);
     uint64_t i = base + t.rank();

@@ -85,6 +85,7 @@ struct write_ctx {
   uint32_t cur;    // write stamp (tile-uniform)
   uint32_t want;   // writes this tile still has queued (caps refills)
   uint32_t grow;   // next refill size: starts at 2, doubles up to W
+  uint32_t exact;  // nonzero: `want` is exact, version-copy refills take want + 2
   uint32_t backoff;  // sleep rounds after the next failed lock (tile-uniform)
 };
 
@@ -135,6 +136,7 @@ struct tree_view : tree_ptrs {
     c.cur = refresh_stamp(t);
     c.want = NW;
     c.grow = 2;
+    c.exact = 0;
     c.backoff = 1;
     return c;
   }
@@ -149,19 +151,24 @@ struct tree_view : tree_ptrs {
   // and slab_alloc.hpp:131-205). Refills take W recycled indices from the free
   // stack, the rest from the never-used frontier. Returns kNone when the pool
   // is exhausted.
-  // Refill size: grows geometrically (2, 4, 8, ... W) with the tile's actual
-  // demand, so a tile never holds more than about twice what it has used, and
-  // is capped by the writes it still has queued (+2 for splits). A plain tree
-  // writes in place and allocates only to split (2 for the root).
-  static constexpr long long kAmpleSlots = 1ll << 20;
-  MVBT_DEV uint32_t refill_size(write_ctx& c) const {
+  // Refill size. A version copy in a write call that knows its queued writes
+  // exactly (c.exact) takes want + 2 at once: in a new epoch nearly every
+  // write copies its leaf, and one atomic on the shared counter then serves
+  // the whole chunk. Otherwise (splits, and callers that hold one operation
+  // at a time) refills grow geometrically (2, 4, 8, ... W) with demand, so a
+  // tile never parks much more than it has used: with many tiles, parked
+  // slots could otherwise exhaust a nearly full pool. A plain tree writes in
+  // place and allocates only to split (2 for the root). Capped at W.
+  MVBT_DEV uint32_t refill_size(write_ctx& c, bool copy) const {
     uint32_t k = VER ? c.want + 2 : 2;
-    if (k > c.grow) k = c.grow;
-    if (c.grow < (uint32_t)NW) c.grow *= 2;
+    if (!(copy && c.exact)) {
+      if (k > c.grow) k = c.grow;
+      if (c.grow < (uint32_t)NW) c.grow *= 2;
+    }
     return k < (uint32_t)NW ? k : (uint32_t)NW;
   }
-  MVBT_DEV void refill(write_ctx& c, const tile_t& t) const {
-    const uint32_t k = refill_size(c);
+  MVBT_DEV void refill(write_ctx& c, bool copy, const tile_t& t) const {
+    const uint32_t k = refill_size(c, copy);
     long long top = 0;
     unsigned long long b = 0;
     if (t.leader()) {
@@ -185,15 +192,9 @@ struct tree_view : tree_ptrs {
     c.mag = idx;
     c.mag_n = (uint32_t)__builtin_popcountll(ok);
     if (c.mag_n < k) flag(kErrPoolExhausted, t);
-    // Refills start small so that slots parked unused in many magazines cannot
-    // exhaust a nearly full pool; once a refill shows ample headroom, the tile
-    // takes full magazines (one atomic on the shared counter per W slots).
-    const bool ample = top - (long long)k > kAmpleSlots ||
-                       (got < k && b + k + (unsigned long long)kAmpleSlots < (unsigned long long)cap_);
-    if (ample) c.grow = NW;
   }
-  MVBT_DEV uint32_t alloc(write_ctx& c, const tile_t& t) const {
-    if (c.mag_n == 0) refill(c, t);
+  MVBT_DEV uint32_t alloc(write_ctx& c, const tile_t& t, bool copy = false) const {
+    if (c.mag_n == 0) refill(c, copy, t);
     if (c.mag_n == 0) return kNone;
     c.mag_n -= 1;
     return t.shfl(c.mag, c.mag_n);
@@ -494,7 +495,7 @@ struct tree_view : tree_ptrs {
     if constexpr (VER) {
       uint64_t ver = n.version(t);
       if (node_t::stamp(ver) != c.cur) {
-        uint32_t cp = alloc(c, t);
+        uint32_t cp = alloc(c, t, true);
         if (cp == kNone) return false;
         n.store_unlocked_copy(at(cp), t);
         retire(c, cp, t);
