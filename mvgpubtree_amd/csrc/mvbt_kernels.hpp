@@ -105,11 +105,16 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
     uint32_t key = active ? keys[i] : 0u;
     uint32_t val = (active && OP == 0) ? values[i] : 0u;
     uint64_t q = t.ballot(active);
+    // Route the whole chunk first (one latency per level for all its keys),
+    // then write key by key starting from the routed leaf.
+    uint32_t leaf = kNone, par = kNone;
+    tv.template descend_chunk<true>(key, q, leaf, par, t);
     while (q) {
       uint32_t j = (uint32_t)lowest_bit(q);
       ctx.want = (uint32_t)__builtin_popcountll(q);
       q &= q - 1;
-      tv.template write<OP>(t.shfl(key, j), OP == 0 ? t.shfl(val, j) : 0u, ctx, t);
+      tv.template write<OP>(t.shfl(key, j), OP == 0 ? t.shfl(val, j) : 0u, ctx, t, t.shfl(leaf, j),
+                            t.shfl(par, j));
     }
   }
   tv.finish(ctx, t);

@@ -167,6 +167,29 @@ def test_config1_plain_full_size(dev):
     tree.close()
 
 
+@pytest.mark.parametrize("kind", ["plain", "versioned"])
+def test_large_single_launch_build(dev, kind):
+    """8M random keys into an empty tree in one insert launch: the tree grows
+    from one leaf while the launch runs, so routing done early in a chunk goes
+    stale by millions of leaves; writes must still land (no step bound, no
+    error flag) and every key must be found with its value."""
+    rng = np.random.default_rng(21)
+    keys = wl.unique_keys(rng, 8_000_000)
+    vals = wl.value_of(keys)
+    cls = mv.GpuBlinkTree if kind == "plain" else mv.GpuVersionedBTree
+    tree = cls(pool_nodes=1 << 21)
+    tree.insert(to_dev(keys, dev), to_dev(vals, dev))
+    sync()
+    assert_clean(tree)
+    q = np.concatenate([keys[::8], wl.absent_keys(rng, 500_000, np.sort(keys))])
+    got = to_np(tree.find(to_dev(q, dev)))
+    want = oracle.SortedState(keys, vals).find(q)
+    assert np.array_equal(got, want)
+    nk, _ = tree.validate_tree_structure()
+    assert nk == keys.size
+    tree.close()
+
+
 @pytest.mark.parametrize("nw", WIDTHS)
 def test_concurrent_snapshot_readers_beside_writers(dev, nw):
     """Finds and ranges at snapshot s run on one stream while inserts and
