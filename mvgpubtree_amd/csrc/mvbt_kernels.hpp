@@ -109,6 +109,10 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
     // then write key by key starting from the routed leaf.
     uint32_t leaf = kNone, par = kNone;
     tv.template descend_chunk<true>(key, q, leaf, par, t);
+    // Leaves that take keys of this chunk are written together; the keys it
+    // hands back (lock busy, split needed, leaf moved) go one at a time.
+    uint32_t res = 0;
+    q = tv.template write_chunk<OP>(key, val, q, leaf, res, ctx, t);
     while (q) {
       uint32_t j = (uint32_t)lowest_bit(q);
       ctx.want = (uint32_t)__builtin_popcountll(q);

@@ -725,6 +725,161 @@ struct tree_view : tree_ptrs {
     }
   }
 
+  // Batched leaf phase for a routed chunk (lane j: key j, value j, routed
+  // leaf j; `q` the keys to write). The leaves that receive the chunk's keys
+  // are handled kLeafBatch at a time, each step for all of them before the
+  // next: lock (one lane-parallel atomic), reload, apply the keys in
+  // registers in lane order, store the version copies, one wait, store the
+  // version words (lane-parallel), one wait, store the data lanes, one wait,
+  // unlock (lane-parallel). Per leaf this is the order write() uses
+  // (copy -> version word -> data -> unlock), so snapshot readers validate
+  // the same way; a chunk pays about six waits per batch of leaves instead
+  // of six per key. Keys whose leaf was not locked, would overflow, or no
+  // longer covers the key are returned as a mask for write(); lane j's
+  // `result` receives kWriteNew / kWriteSame for the others.
+  static constexpr int kLeafBatch = 8;
+  template <int OP>
+  MVBT_DEV uint64_t write_chunk(uint32_t key, uint32_t value, uint64_t q, uint32_t leaf,
+                                uint32_t& result, write_ctx& c, const tile_t& t) const {
+    if constexpr (NW > 16) {
+      return q;  // wide nodes: register budget; per-key path
+    } else {
+      const uint32_t r = t.rank();
+      const uint64_t todo = q & t.ballot(valid_key(key) && leaf < cap_);
+      uint64_t left = q & ~todo;
+      // group keys by leaf: my_lead = lowest lane routed to the same leaf
+      uint64_t lead = 0;
+      uint32_t my_lead = 0;
+      for (uint64_t m = todo; m;) {
+        const uint32_t j = (uint32_t)lowest_bit(m);
+        const uint64_t same = todo & t.ballot(leaf == t.shfl(leaf, j));
+        lead |= 1ull << j;
+        if ((same >> r) & 1) my_lead = j;
+        m &= ~same;
+      }
+      for (uint64_t pending = lead; pending;) {
+        uint64_t batch = 0;
+        {
+          uint64_t m = pending;
+          for (int i = 0; i < kLeafBatch && m; ++i) {
+            batch |= m & (~m + 1);
+            m &= m - 1;
+          }
+        }
+        pending &= ~batch;
+        // 1. lock every leaf of the batch (each leader lane its own leaf)
+        uint32_t old = kLockBit;
+        if ((batch >> r) & 1) old = atomic_or32(lock_word(leaf), kLockBit);
+        const uint64_t got = batch & t.ballot(!(old & kLockBit));
+        for (uint64_t m = batch & ~got; m; m &= m - 1)
+          left |= todo & t.ballot(my_lead == (uint32_t)lowest_bit(m));
+        // 2. reload the locked leaves
+        uint64_t img[kLeafBatch];
+        uint32_t gl[kLeafBatch];
+        {
+          uint64_t m = got;
+#pragma unroll
+          for (int i = 0; i < kLeafBatch; ++i) {
+            gl[i] = m ? (uint32_t)lowest_bit(m) : 0u;
+            if (m) img[i] = ld_coherent(at(t.shfl(leaf, gl[i])) + r);
+            m &= m - 1;
+          }
+        }
+        const int ng = __builtin_popcountll(got);
+        // 3. plan: apply each leaf's keys in registers; a leaf whose keys do
+        // not all fit (or that no longer covers one of them) goes per key
+        uint32_t ok = 0, changed = 0, need_copy = 0;  // bit i: slot i
+#pragma unroll
+        for (int i = 0; i < kLeafBatch; ++i) {
+          if (i >= ng) continue;
+          node_t n;
+          n.p = img[i];
+          const uint64_t meta = n.meta(t);
+          const uint64_t mem = todo & t.ballot(my_lead == gl[i]);
+          bool good = node_t::is_leaf(meta);
+          bool chg = false;
+          for (uint64_t mm = mem; mm && good; mm &= mm - 1) {
+            const uint32_t j = (uint32_t)lowest_bit(mm);
+            const uint32_t kj = t.shfl(key, j);
+            if (kj >= node_t::high_key(meta)) {
+              good = false;
+            } else if (OP == 0) {
+              if (!n.contains(kj, t) && n.is_full(t)) {
+                good = false;
+              } else {
+                const bool fresh = n.insert(kj, t.shfl(value, j), t);
+                if (r == j) result = fresh ? kWriteNew : kWriteSame;
+                chg = true;
+              }
+            } else {
+              const bool had = n.erase(kj, t);
+              if (r == j) result = had ? kWriteNew : kWriteSame;
+              chg |= had;
+            }
+          }
+          if (!good) {
+            left |= mem;
+          } else {
+            ok |= 1u << i;
+            if (chg) changed |= 1u << i;
+            node_t o;
+            o.p = img[i];
+            if (VER && chg && node_t::stamp(o.version(t)) != c.cur) need_copy |= 1u << i;
+          }
+        }
+        // 4. version copies (new epoch), then their version words
+        if constexpr (VER) {
+          uint32_t cp = kNone;  // leader lane: copy slot of its leaf
+#pragma unroll
+          for (int i = 0; i < kLeafBatch; ++i) {
+            if (!((need_copy >> i) & 1)) continue;
+            const uint32_t slot = alloc(c, t, true);
+            if (slot == kNone) {  // pool exhausted: leave the leaf unchanged
+              ok &= ~(1u << i);
+              changed &= ~(1u << i);
+              need_copy &= ~(1u << i);
+              continue;
+            }
+            node_t n;
+            n.p = img[i];
+            n.store_unlocked_copy(at(slot), t);
+            retire(c, slot, t);
+            if (r == gl[i]) cp = slot;
+          }
+          if (need_copy) {
+            order_release();
+            if (cp != kNone) st_coherent(at(leaf) + node_t::kVerLane, pack(c.cur, cp));
+            order_release();
+          }
+        }
+        // 5. data lanes (the plan re-applied to the loaded image)
+#pragma unroll
+        for (int i = 0; i < kLeafBatch; ++i) {
+          if (!((changed >> i) & 1)) continue;
+          node_t n;
+          n.p = img[i];
+          const uint64_t mem = todo & t.ballot(my_lead == gl[i]);
+          for (uint64_t mm = mem; mm; mm &= mm - 1) {
+            const uint32_t j = (uint32_t)lowest_bit(mm);
+            if (OP == 0) n.insert(t.shfl(key, j), t.shfl(value, j), t);
+            else n.erase(t.shfl(key, j), t);
+          }
+          // (the shuffle must run on every lane of the tile: a permute reads
+          // nothing from a disabled source lane, and the leader may be one)
+          const uint32_t target = t.shfl(leaf, gl[i]);
+          if (node_t::data_lane(r)) st_coherent(at(target) + r, n.p);
+        }
+        // 6. unlock every locked leaf
+        order_release();
+        if ((got >> r) & 1) atomic_and32(lock_word(leaf), ~kLockBit);
+        // keys of leaves that failed for lack of memory go per key (they fail there too)
+        for (int i = 0; i < ng; ++i)
+          if (!((ok >> i) & 1)) left |= todo & t.ballot(my_lead == gl[i]);
+      }
+      return left;
+    }
+  }
+
   MVBT_DEV uint32_t insert(uint32_t key, uint32_t value, write_ctx& c, const tile_t& t) const {
     return write<0>(key, value, c, t);
   }
