@@ -425,6 +425,110 @@ struct tree_view : tree_ptrs {
     parent = par;
   }
 
+  // Pass-based reads of a chunk (for launches in which no writer runs): each
+  // key of lane j is a small state machine (route -> leaf -> version chain ->
+  // leaf scan), and every pass loads one node for each unfinished key of a
+  // group before it uses any of them. A group of kGroup keys then waits about
+  // one latency per level, version hop and scanned leaf, instead of one per
+  // key and step. RANGE = false: find of key `lo`; returns the value visible
+  // at `ts` or kEmpty. RANGE = true: range [lo, hi) with lo >= 1; pairs of the
+  // key in lane j go to out + j * cap (at most cap of them; out may be null)
+  // and the true count is returned. Keys outside `todo` return kEmpty / 0.
+  static constexpr int kReadGroup = NW < 8 ? NW : 8;
+  template <bool RANGE>
+  MVBT_DEV uint32_t read_chunk(uint32_t lo, uint32_t hi, uint32_t ts, uint64_t* out, uint32_t cap,
+                               uint64_t todo, const tile_t& t) const {
+    enum : uint32_t { kRoute = 0, kArrive = 1, kChain = 2 };
+    const uint32_t r = t.rank();
+    uint32_t cur = root_, phase = kRoute, cnt = 0;
+    uint32_t res = RANGE ? 0u : kEmpty;
+    uint32_t err = 0;
+    todo &= t.ballot(valid_key(lo) && (!RANGE || hi > lo));
+    for (int base = 0; base < NW; base += kReadGroup) {
+      uint64_t active = (todo >> base) & ((kReadGroup == 64) ? ~0ull : ((1ull << kReadGroup) - 1));
+      for (uint32_t pass = 0; active != 0; ++pass) {
+        if (pass >= kMaxReadSteps) {
+          err |= kErrCorrupt;
+          break;
+        }
+        uint64_t p[kReadGroup];
+#pragma unroll
+        for (int j = 0; j < kReadGroup; ++j) {
+          const uint32_t idx = t.shfl(cur, (uint32_t)(base + j));
+          if ((active >> j) & 1) p[j] = ld_weak(at(idx) + r);
+        }
+#pragma unroll
+        for (int j = 0; j < kReadGroup; ++j) {
+          if (!((active >> j) & 1)) continue;
+          const uint32_t lj = (uint32_t)(base + j);
+          node_t n;
+          n.p = p[j];
+          const uint64_t m = n.meta(t);
+          const uint32_t k = t.shfl(lo, lj);
+          uint32_t ph = t.shfl(phase, lj);
+          uint32_t next = kNone;
+          bool visit = false, done = false;
+          if (ph == kRoute) {
+            if (k >= node_t::high_key(m)) next = node_t::sibling(m);
+            else if (!node_t::is_leaf(m)) next = n.child_for(k, t);
+            else ph = kArrive;
+          }
+          if (ph == kArrive || ph == kChain) {
+            uint64_t ver = 0;
+            bool later = false;
+            if constexpr (VER) {
+              ver = n.version(t);
+              later = ts != kLatest && node_t::stamp(ver) > ts;
+            }
+            if (later) {
+              next = node_t::older(ver);
+              ph = kChain;
+              if (next == kNone) {  // no version old enough
+                err |= kErrVersionGap;
+                done = true;
+              }
+            } else {
+              visit = true;
+            }
+          }
+          uint32_t c = 0;
+          if (visit) {
+            if constexpr (RANGE) {
+              const uint32_t h = t.shfl(hi, lj);
+              c = t.shfl(cnt, lj);
+              c += n.collect(k, h, out ? out + (size_t)lj * cap : nullptr, c, cap, t);
+              if (h <= node_t::high_key(m) || node_t::sibling(m) == kNoSibling) {
+                done = true;
+              } else {
+                next = node_t::sibling(m);
+                ph = kArrive;
+              }
+            } else {
+              const uint32_t v = n.lookup(k, t);
+              if (r == lj) res = v;
+              done = true;
+            }
+          }
+          if (!done && next >= cap_) {
+            err |= kErrCorrupt;
+            done = true;
+          }
+          if (r == lj) {
+            if (RANGE && visit) cnt = c;
+            if (!done) {
+              cur = next;
+              phase = ph;
+            }
+          }
+          if (done) active &= ~(1ull << j);
+        }
+      }
+    }
+    if (RANGE) res = cnt;
+    if (err) flag(err, t);
+    return res;
+  }
+
   // ---- find -----------------------------------------------------------------
   template <bool CONC>
   MVBT_DEV uint32_t find(uint32_t key, uint32_t ts, const tile_t& t) const {
