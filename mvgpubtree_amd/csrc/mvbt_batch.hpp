@@ -139,6 +139,27 @@ __global__ void batch_finish_kernel(batch_job* job) {
   if (threadIdx.x == 0 && blockIdx.x == 0 && job->base_out) *job->base_out = job->clock0;
 }
 
+// Batch of op i for a caller whose previous op was in batch `from` (op
+// indices only grow for a tile): gallop forward from `from`, then bisect, so
+// the usual case (same or next batch) costs one or two loads instead of
+// log2(nb).
+MVBT_DEV uint32_t batch_after(const unsigned long long* offsets, uint32_t nb, unsigned long long i, uint32_t from) {
+  uint32_t lo = from, step = 1;  // offsets[lo] <= i
+  uint32_t hi = lo + step;
+  while (hi < nb && offsets[hi] <= i) {
+    lo = hi;
+    step <<= 1;
+    hi = lo + step;
+  }
+  if (hi > nb) hi = nb;  // offsets[lo] <= i < offsets[hi]
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (offsets[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 MVBT_DEV uint32_t batch_of(const unsigned long long* offsets, uint32_t nb, unsigned long long i) {
   uint32_t lo = 0, hi = nb;  // offsets[lo] <= i < offsets[hi] (i in [offsets[0], offsets[nb]))
   while (hi - lo > 1) {
@@ -161,6 +182,7 @@ MVBT_DEV uint32_t batch_of(const unsigned long long* offsets, uint32_t nb, unsig
 template <int NW>
 __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_job* job) {
   const tree_view<NW, true> tv(tp);
+  using node_t = node<NW, true>;
   tile<NW> t;
   const unsigned long long nops = job->nops;
   const uint32_t nb = job->nb, L = job->L, clock0 = job->clock0;
@@ -196,7 +218,7 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
         idx = base + (unsigned long long)__builtin_popcountll(below);
         if (idx >= nops) break;  // queue drained
         op = ops[idx];
-        b = batch_of(job->offsets, nb, idx);
+        b = batch_after(job->offsets, nb, idx, b);
         have = true;
       }
     }
@@ -225,19 +247,29 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
     }
     waits = 0;
     backoff = 1;
+    // Every operation starts with a root-to-leaf walk, so the tiles of a wave
+    // take it together whatever their operation types: their node loads
+    // overlap instead of running type by type. Only the short leaf-side parts
+    // below diverge.
+    uint32_t rkey = op.key;
+    if (op.type == kOpRange && rkey == 0) rkey = 1;
+    const bool route = valid_key(rkey) && !(op.type == kOpRange && op.arg <= rkey);
+    node_t n;
+    uint32_t leaf = kNone, par = kNone;
+    if (route) leaf = tv.template descend<true, true>(rkey, n, t, &par);
     uint32_t r;
     if (op.type == kOpFind) {
-      r = tv.template find<true>(op.key, clock0 + b, t);
+      r = route ? tv.template find_in<true>(op.key, clock0 + b, n, leaf, t) : kEmpty;
     } else if (op.type == kOpRange) {
       uint64_t* out = (pairs && op.slot < pair_slots) ? pairs + (size_t)op.slot * L : nullptr;
       if (pairs && op.slot >= pair_slots) tv.flag(kErrCorrupt, t);
-      r = tv.template range<true>(op.key, op.arg, clock0 + b, out, L, t);
+      r = route ? tv.template range_in<true>(rkey, op.arg, clock0 + b, n, leaf, out, L, t) : 0u;
       if (r > L) tv.flag(kErrRangeOverflow, t);
     } else {
       ctx.cur = clock0 + b + 1;
       ctx.want = NW;  // refills grow with demand (write_ctx::grow)
-      r = (op.type == kOpInsert) ? tv.template write<0>(op.key, op.arg, ctx, t)
-                                 : tv.template write<1>(op.key, 0u, ctx, t);
+      r = (op.type == kOpInsert) ? tv.template write<0>(op.key, op.arg, ctx, t, leaf, par)
+                                 : tv.template write<1>(op.key, 0u, ctx, t, leaf, par);
       order_release();  // this write's stores complete before it is counted
       if (t.leader()) {
         uint32_t prev = atomic_add32(job->done + b, 1u);

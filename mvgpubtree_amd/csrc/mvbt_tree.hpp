@@ -336,19 +336,32 @@ struct tree_view : tree_ptrs {
 
   // Descends (latest routing, B-link right moves) to the leaf covering `key`.
   // Returns kNone (and flags kErrCorrupt) if the walk leaves the pool.
-  template <bool CONC>
-  MVBT_DEV uint32_t descend(uint32_t key, node_t& n, const tile_t& t) const {
-    uint32_t idx = root_;
+  // `parent` (optional) receives the internal node the leaf was reached from
+  // (kNone after a right move at the leaf level). CACHED_ROUTE (with CONC):
+  // internal nodes are read through the L1, where the top levels stay hot
+  // (routing only: a stale image routes too far left and the right moves
+  // repair it), and a node found to be a leaf is read again coherently.
+  template <bool CONC, bool CACHED_ROUTE = false>
+  MVBT_DEV uint32_t descend(uint32_t key, node_t& n, const tile_t& t, uint32_t* parent = nullptr) const {
+    uint32_t idx = root_, par = kNone;
     for (uint32_t step = 0; step < kMaxReadSteps; ++step) {
-      load<CONC>(n, idx, t);
+      if constexpr (CONC && CACHED_ROUTE) {
+        n.load_weak(at(idx), t);
+        if (node_t::is_leaf(n.meta(t))) n.load(at(idx), t);
+      } else {
+        load<CONC>(n, idx, t);
+      }
       uint64_t m = n.meta(t);
       uint32_t next;
       if (key >= node_t::high_key(m)) {
         next = node_t::sibling(m);
+        par = kNone;
       } else if (node_t::is_leaf(m)) {
+        if (parent) *parent = par;
         return idx;
       } else {
         next = n.child_for(key, t);
+        par = idx;
       }
       if (next >= cap_) break;
       idx = next;
@@ -535,6 +548,11 @@ struct tree_view : tree_ptrs {
     if (!valid_key(key)) return kEmpty;
     node_t n;
     uint32_t idx = descend<CONC>(key, n, t);
+    return find_in<CONC>(key, ts, n, idx, t);
+  }
+  // The find once `key` has been routed to leaf `idx`, loaded in `n`.
+  template <bool CONC>
+  MVBT_DEV uint32_t find_in(uint32_t key, uint32_t ts, node_t& n, uint32_t idx, const tile_t& t) const {
     if (idx == kNone || !to_version<CONC>(n, idx, ts, t)) return kEmpty;
     return n.lookup(key, t);
   }
@@ -548,6 +566,13 @@ struct tree_view : tree_ptrs {
     if (hi <= lo) return 0;
     node_t n;
     uint32_t idx = descend<CONC>(lo, n, t);
+    return range_in<CONC>(lo, hi, ts, n, idx, out, cap, t);
+  }
+  // The range query once `lo` (>= 1, below hi) has been routed to leaf `idx`,
+  // loaded in `n`.
+  template <bool CONC>
+  MVBT_DEV uint32_t range_in(uint32_t lo, uint32_t hi, uint32_t ts, node_t& n, uint32_t idx, uint64_t* out,
+                             uint32_t cap, const tile_t& t) const {
     if (idx == kNone || !to_version<CONC>(n, idx, ts, t)) return 0;
     uint32_t count = 0;
     for (uint32_t step = 0;; ++step) {
