@@ -82,6 +82,21 @@ class tree_base {
                            concurrent ? MVBT_CONCURRENT : 0),
           "range_query");
   }
+  // Reference concurrent_insert_range / concurrent_find_erase: one launch per
+  // call; on versioned trees the reads see the snapshot taken at the call.
+  void concurrent_insert_range(const key_type* keys, const value_type* values, size_type num_insertion,
+                               const key_type* lower_bound, const key_type* upper_bound, size_type num_ranges,
+                               pair_type* result, size_type average_range_length, hipStream_t stream = 0,
+                               size_type* counts = nullptr) {
+    check(mvbt_concurrent_insert_range(h_, keys, values, num_insertion, lower_bound, upper_bound, num_ranges,
+                                       reinterpret_cast<mvbt_pair*>(result), counts, average_range_length, stream),
+          "concurrent_insert_range");
+  }
+  void concurrent_find_erase(const key_type* find_keys, value_type* find_results, size_type num_finds,
+                             const key_type* erase_keys, size_type num_erasures, hipStream_t stream = 0) {
+    check(mvbt_concurrent_find_erase(h_, find_keys, find_results, num_finds, erase_keys, num_erasures, stream),
+          "concurrent_find_erase");
+  }
   // Sorted bulk load of this new tree (fill 0 = default entries per node).
   void bulk_build(const key_type* keys, const value_type* values, uint64_t num_keys, uint32_t fill = 0,
                   hipStream_t stream = 0) {

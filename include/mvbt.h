@@ -177,6 +177,31 @@ mvbt_status mvbt_export(mvbt_tree* tree, mvbt_pair* out, uint64_t capacity, uint
 mvbt_status mvbt_bulk_build(mvbt_tree* tree, const uint32_t* keys, const uint32_t* values, uint64_t n,
                             uint32_t fill, void* stream);
 
+/* Replaces concurrent_insert_range(keys, values, num_insertion, lower_bound,
+ * upper_bound, num_ranges, result, average_range_length, stream)
+ * (gpu_versioned_blink_tree.hpp:250-294, gpu_blink_tree.hpp:148-187): inserts
+ * and range queries of one call run in one launch. Versioned trees: the range
+ * queries read the snapshot taken at the start of the call (the clock advances
+ * by one; the inserts are stamped after it), so their results are
+ * deterministic (the reference snapshots inside the kernel). Plain trees: the
+ * queries see the latest state (any linearization, as in the reference). Range
+ * i writes at most L pairs to result[i*L ...] (result may be NULL) and its true
+ * count to counts[i] (counts may be NULL). Calls on one handle must not
+ * overlap on different streams. */
+mvbt_status mvbt_concurrent_insert_range(mvbt_tree* tree, const uint32_t* keys, const uint32_t* values,
+                                         uint64_t num_insertions, const uint32_t* lower, const uint32_t* upper,
+                                         uint64_t num_ranges, mvbt_pair* result, uint32_t* counts, uint32_t L,
+                                         void* stream);
+
+/* Replaces concurrent_find_erase(find_keys, find_results, num_finds, erase_keys,
+ * num_erasures, stream) (gpu_versioned_blink_tree.hpp:218-248,
+ * gpu_blink_tree.hpp:189-224): finds and erases of one call run in one launch;
+ * snapshot semantics as for mvbt_concurrent_insert_range (versioned erases are
+ * versioned). */
+mvbt_status mvbt_concurrent_find_erase(mvbt_tree* tree, const uint32_t* find_keys, uint32_t* find_results,
+                                       uint64_t num_finds, const uint32_t* erase_keys, uint64_t num_erasures,
+                                       void* stream);
+
 /* ---- Batched form: many batches of mixed operations in one launch ---------- */
 
 #define MVBT_OP_FIND 0u

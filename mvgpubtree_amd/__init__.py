@@ -49,7 +49,8 @@ EXPORTED_SYMBOLS = (
     "mvbt_range_query", "mvbt_take_snapshot", "mvbt_release_snapshot", "mvbt_reclaim",
     "mvbt_memory_stats", "mvbt_error_flags", "mvbt_clear_error_flags", "mvbt_validate",
     "mvbt_export", "mvbt_node_width", "mvbt_kind_of", "mvbt_run_batches", "mvbt_set_worker_waves",
-    "mvbt_memory_stats_async", "mvbt_device_view", "mvbt_bulk_build",
+    "mvbt_memory_stats_async", "mvbt_device_view", "mvbt_bulk_build", "mvbt_concurrent_insert_range",
+    "mvbt_concurrent_find_erase",
 )
 
 
@@ -136,6 +137,8 @@ def load_library():
         "mvbt_memory_stats_async": ([vp, vp, vp], i32),
         "mvbt_device_view": ([vp, vp], i32),
         "mvbt_bulk_build": ([vp, vp, vp, u64, u32, vp], i32),
+        "mvbt_concurrent_insert_range": ([vp, vp, vp, u64, vp, vp, u64, vp, vp, u32, vp], i32),
+        "mvbt_concurrent_find_erase": ([vp, vp, vp, u64, vp, u64, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -244,6 +247,39 @@ class _TreeBase:
             n = keys.numel()
         _check(load_library().mvbt_bulk_build(self._h, _ptr(keys, 4, "keys"), _ptr(values, 4, "values"),
                                               n, fill, _stream(stream)), "bulk_build")
+
+    def concurrent_insert_range(self, keys, values, lower, upper, average_range_length: int,
+                                result=None, counts=None, stream=None):
+        """Inserts and range queries in one launch (reference concurrent_insert_range).
+        Returns (pairs int64[num_ranges * L], counts int32[num_ranges])."""
+        torch = _torch()
+        n_i, n_r, L = keys.numel(), lower.numel(), average_range_length
+        if values.numel() != n_i or upper.numel() != n_r:
+            raise ValueError("keys/values or lower/upper differ in length")
+        dev = (keys if n_i else lower).device
+        if result is None:
+            result = torch.full((max(n_r, 1) * L,), -1, dtype=torch.int64, device=dev)
+        if counts is None:
+            counts = torch.empty(max(n_r, 1), dtype=torch.int32, device=dev)
+        if result.numel() < n_r * L or counts.numel() < n_r:
+            raise ValueError("result or counts too short")
+        _check(load_library().mvbt_concurrent_insert_range(
+            self._h, _ptr(keys, 4, "keys") if n_i else None, _ptr(values, 4, "values") if n_i else None, n_i,
+            _ptr(lower, 4, "lower") if n_r else None, _ptr(upper, 4, "upper") if n_r else None, n_r,
+            _ptr(result, 8, "result"), _ptr(counts, 4, "counts"), L, _stream(stream)), "concurrent_insert_range")
+        return result, counts
+
+    def concurrent_find_erase(self, find_keys, erase_keys, find_results=None, stream=None):
+        """Finds and erases in one launch (reference concurrent_find_erase). Returns find results."""
+        torch = _torch()
+        n_f, n_e = find_keys.numel(), erase_keys.numel()
+        if find_results is None:
+            find_results = torch.empty(max(n_f, 1), dtype=torch.int32,
+                                       device=(find_keys if n_f else erase_keys).device)
+        _check(load_library().mvbt_concurrent_find_erase(
+            self._h, _ptr(find_keys, 4, "find_keys") if n_f else None, _ptr(find_results, 4, "find_results"), n_f,
+            _ptr(erase_keys, 4, "erase_keys") if n_e else None, n_e, _stream(stream)), "concurrent_find_erase")
+        return find_results
 
     def insert(self, keys, values, stream=None):
         n = keys.numel()

@@ -167,6 +167,17 @@ struct impl {
     return MVBT_OK;
   }
   static constexpr uint32_t data_lanes() { return node<NW, VER>::kData; }
+  template <bool IR>
+  static mvbt_status mixed(mvbt_tree* t, const uint32_t* wk, const uint32_t* wv, uint64_t nw, const uint32_t* rlo,
+                           const uint32_t* rhi, uint64_t nr, uint32_t* fout, uint64_t* pairs, uint32_t* counts,
+                           uint32_t L, hipStream_t s) {
+    if (VER) hipLaunchKernelGGL(mixed_setup_kernel, dim3(1), dim3(64), 0, s, t->st);
+    uint32_t g = grid_for(t, nw + nr, NW, 32);
+    hipLaunchKernelGGL((mixed_kernel<NW, VER, IR>), dim3(g), dim3(kBlock), 0, s, t->template view<NW, VER>(), wk, wv,
+                       nw, rlo, rhi, nr, fout, pairs, counts, L);
+    MVBT_HIP(hipGetLastError());
+    return MVBT_OK;
+  }
 };
 
 template <int NW>
@@ -184,13 +195,18 @@ struct dispatch_table {
   void* (*worker)();
   mvbt_status (*bulk)(mvbt_tree*, const uint32_t*, const uint32_t*, const bulk_plan&, hipStream_t);
   uint32_t data_lanes;
+  using mixed_fn = mvbt_status (*)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, const uint32_t*,
+                                   const uint32_t*, uint64_t, uint32_t*, uint64_t*, uint32_t*, uint32_t, hipStream_t);
+  mixed_fn insert_range;
+  mixed_fn find_erase;
 };
 
 template <int NW, bool VER>
 dispatch_table make_table() {
   using I = impl<NW, VER>;
   return dispatch_table{&I::init, &I::find, &I::range, &I::template write<0>, &I::template write<1>,
-                        VER ? &worker_fn<NW> : nullptr, &I::bulk, I::data_lanes()};
+                        VER ? &worker_fn<NW> : nullptr, &I::bulk, I::data_lanes(),
+                        &I::template mixed<true>, &I::template mixed<false>};
 }
 
 const dispatch_table* table_for(uint32_t nw, bool versioned) {
@@ -803,6 +819,44 @@ mvbt_status mvbt_device_view(mvbt_tree* t, mvbt_device_handle* out) {
   t->external_writers = true;
   t->fresh = false;
   return MVBT_OK;
+}
+
+mvbt_status mvbt_concurrent_insert_range(mvbt_tree* t, const uint32_t* keys, const uint32_t* values,
+                                         uint64_t num_insertions, const uint32_t* lower, const uint32_t* upper,
+                                         uint64_t num_ranges, mvbt_pair* result, uint32_t* counts, uint32_t L,
+                                         void* stream) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (num_insertions + num_ranges == 0) return MVBT_OK;
+  if (num_insertions && (!keys || !values)) return fail(MVBT_ERR_INVALID_ARG, "null keys or values");
+  if (num_ranges && (!lower || !upper)) return fail(MVBT_ERR_INVALID_ARG, "null range bounds");
+  if (num_ranges && result && L == 0) return fail(MVBT_ERR_INVALID_ARG, "L must be > 0 when result is given");
+  device_guard g(t->device);
+  std::lock_guard<std::mutex> lk(t->mu);
+  hipStream_t s = (hipStream_t)stream;
+  if (mvbt_status rc = before_write(t, s)) return rc;
+  if (mvbt_status rc = table_for(t->nw, t->versioned)
+                           ->insert_range(t, keys, values, num_insertions, lower, upper, num_ranges, nullptr,
+                                          reinterpret_cast<uint64_t*>(result), counts, L, s))
+    return rc;
+  return after_write(t, s);
+}
+
+mvbt_status mvbt_concurrent_find_erase(mvbt_tree* t, const uint32_t* find_keys, uint32_t* find_results,
+                                       uint64_t num_finds, const uint32_t* erase_keys, uint64_t num_erasures,
+                                       void* stream) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (num_finds + num_erasures == 0) return MVBT_OK;
+  if (num_finds && (!find_keys || !find_results)) return fail(MVBT_ERR_INVALID_ARG, "null find keys or results");
+  if (num_erasures && !erase_keys) return fail(MVBT_ERR_INVALID_ARG, "null erase keys");
+  device_guard g(t->device);
+  std::lock_guard<std::mutex> lk(t->mu);
+  hipStream_t s = (hipStream_t)stream;
+  if (mvbt_status rc = before_write(t, s)) return rc;
+  if (mvbt_status rc = table_for(t->nw, t->versioned)
+                           ->find_erase(t, erase_keys, nullptr, num_erasures, find_keys, nullptr, num_finds,
+                                        find_results, nullptr, nullptr, 0, s))
+    return rc;
+  return after_write(t, s);
 }
 
 mvbt_status mvbt_bulk_build(mvbt_tree* t, const uint32_t* keys, const uint32_t* values, uint64_t n,

@@ -133,6 +133,88 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
   tv.finish(ctx, t);
 }
 
+// Mixed launch (reference concurrent_insert_range_kernel btree_kernels.hpp:150-306,
+// concurrent_find_erase_kernel_versioned :31-143, and their blink variants
+// :672-860): the writes and reads of one call run in one grid, chunks taken
+// grid-stride over [writes | reads]. Versioned trees: the reads see the
+// snapshot reserved by mixed_setup_kernel (every write of the call is stamped
+// after it), i.e. the linearization in which every read of the call precedes
+// every write; the reference's reads take snapshots inside the kernel instead
+// and are not deterministic. Plain trees: reads see the latest state, any
+// linearization, as in the reference.
+// IR = true: writes insert (wkeys, wvals), reads are ranges [rlo, rhi) with
+// pairs at pairs + j*L and counts. IR = false: writes erase (wkeys), reads
+// find (rlo) into find_out.
+__global__ void mixed_setup_kernel(tree_state* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    st->call_snapshot = __hip_atomic_fetch_add(&st->clock, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NW, bool VER, bool IR>
+__global__ void __launch_bounds__(256)
+    mixed_kernel(tree_view<NW, VER> tv, const uint32_t* wkeys, const uint32_t* wvals, uint64_t nw,
+                 const uint32_t* rlo, const uint32_t* rhi, uint64_t nr, uint32_t* find_out, uint64_t* pairs,
+                 uint32_t* counts, uint32_t L) {
+  tile<NW> t;
+  chunking<NW> c;
+  const uint64_t n = nw + nr;
+  if (c.first >= n) return;
+  constexpr int OP = IR ? 0 : 1;
+  const uint32_t ts = VER ? ld32_coherent(&tv.st->call_snapshot) : kLatest;
+  write_ctx ctx = tv.begin_writes(t);
+  ctx.exact = 1;
+  if (VER) ctx.cur = ts + 1;
+  bool overflow = false;
+  for (uint64_t base = c.first; base < n; base += c.stride) {
+    const uint64_t i = base + t.rank();
+    const bool is_w = i < nw, is_r = i >= nw && i < n;
+    // writes of the chunk: routed together, leaves written together
+    uint64_t q = t.ballot(is_w);
+    if (q) {
+      const uint32_t key = is_w ? wkeys[i] : 0u;
+      const uint32_t val = (IR && is_w) ? wvals[i] : 0u;
+      uint32_t leaf = kNone, par = kNone, res = 0;
+      tv.template descend_chunk<true>(key, q, leaf, par, t);
+      q = tv.template write_chunk<OP>(key, val, q, leaf, res, ctx, t);
+      while (q) {
+        const uint32_t j = (uint32_t)lowest_bit(q);
+        ctx.want = (uint32_t)__builtin_popcountll(q);
+        q &= q - 1;
+        tv.template write<OP>(t.shfl(key, j), IR ? t.shfl(val, j) : 0u, ctx, t, t.shfl(leaf, j), t.shfl(par, j));
+      }
+    }
+    // reads of the chunk, beside the writes of other tiles
+    q = t.ballot(is_r);
+    if (q) {
+      const uint64_t ri = is_r ? i - nw : 0;
+      const uint32_t lo = is_r ? rlo[ri] : 0u;
+      const uint32_t hi = (IR && is_r) ? rhi[ri] : 0u;
+      uint32_t result = IR ? 0u : kEmpty;
+      while (q) {
+        const uint32_t j = (uint32_t)lowest_bit(q);
+        q &= q - 1;
+        uint32_t r;
+        if constexpr (IR) {
+          const uint64_t rj = base + j - nw;
+          uint64_t* slot = pairs ? pairs + rj * (uint64_t)L : nullptr;
+          r = tv.template range<true>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t);
+        } else {
+          r = tv.template find<true>(t.shfl(lo, j), ts, t);
+        }
+        if (t.rank() == j) result = r;
+      }
+      if (IR) {
+        if (is_r && counts) counts[ri] = result;
+        overflow |= is_r && result > L;
+      } else if (is_r) {
+        find_out[ri] = result;
+      }
+    }
+  }
+  if (t.any(overflow)) tv.flag(kErrRangeOverflow, t);
+  tv.finish(ctx, t);
+}
+
 // take_snapshot_kernel (btree_kernels.hpp:473-478): one lane reads the retire
 // ring position, then advances the clock; both go to `rec`.
 __global__ void snapshot_kernel(tree_state* st, snapshot_record* rec) {

@@ -61,6 +61,8 @@ struct alignas(128) tree_state {
   unsigned long long reclaimed;    // total nodes returned to the free stack
   unsigned long long peak_live;    // max live nodes observed at reclaim/stats
   unsigned long long scratch[4];   // host<->device hand-off words
+  uint32_t call_snapshot;          // snapshot reserved by the current mixed call
+  uint32_t pad_call;
 };
 
 // Result of a snapshot taken on the device: the id and the retire-ring
@@ -550,11 +552,43 @@ struct tree_view : tree_ptrs {
     uint32_t idx = descend<CONC>(key, n, t);
     return find_in<CONC>(key, ts, n, idx, t);
   }
+  // Plain trees rewrite a leaf in place (pairs shift on insert and erase), a
+  // multi-lane store a concurrent reader can observe half done; versioned
+  // leaves are validated through their version word instead. So a concurrent
+  // reader of a plain leaf reads it under the leaf's lock: one attempt per
+  // pass of the caller's loop, never a spin of its own, because the holder may
+  // be another tile of the same wave, which releases within the same pass.
+  MVBT_DEV bool lock_and_load(uint32_t idx, node_t& n, const tile_t& t) const {
+    if (!try_lock(idx, t)) {
+      for (uint32_t i = 0; i < 4; ++i) __builtin_amdgcn_s_sleep(2);
+      return false;
+    }
+    n.load(at(idx), t);
+    return true;
+  }
+
   // The find once `key` has been routed to leaf `idx`, loaded in `n`.
   template <bool CONC>
   MVBT_DEV uint32_t find_in(uint32_t key, uint32_t ts, node_t& n, uint32_t idx, const tile_t& t) const {
-    if (idx == kNone || !to_version<CONC>(n, idx, ts, t)) return kEmpty;
-    return n.lookup(key, t);
+    if constexpr (!VER && CONC) {
+      for (uint32_t step = 0; idx < cap_ && step < kMaxReadSteps; ++step) {
+        if (!lock_and_load(idx, n, t)) continue;  // busy: try again next pass
+        const uint64_t m = n.meta(t);
+        if (key >= node_t::high_key(m)) {  // split meanwhile: the key moved right
+          unlock(idx, t);
+          idx = node_t::sibling(m);
+          continue;
+        }
+        const uint32_t v = n.lookup(key, t);
+        unlock(idx, t);
+        return v;
+      }
+      if (idx != kNone) flag(kErrCorrupt, t);
+      return kEmpty;
+    } else {
+      if (idx == kNone || !to_version<CONC>(n, idx, ts, t)) return kEmpty;
+      return n.lookup(key, t);
+    }
   }
 
   // ---- range query [lo, hi) ------------------------------------------------
@@ -573,6 +607,27 @@ struct tree_view : tree_ptrs {
   template <bool CONC>
   MVBT_DEV uint32_t range_in(uint32_t lo, uint32_t hi, uint32_t ts, node_t& n, uint32_t idx, uint64_t* out,
                              uint32_t cap, const tile_t& t) const {
+    if constexpr (!VER && CONC) {
+      // plain leaves read under lock, one at a time (see lock_and_load)
+      uint32_t count = 0;
+      bool first = true;
+      for (uint32_t step = 0; idx < cap_ && step < kMaxReadSteps; ++step) {
+        if (!lock_and_load(idx, n, t)) continue;  // busy: try again next pass
+        const uint64_t m = n.meta(t);
+        if (first && lo >= node_t::high_key(m)) {  // split meanwhile: lo moved right
+          unlock(idx, t);
+          idx = node_t::sibling(m);
+          continue;
+        }
+        first = false;
+        count += n.collect(lo, hi, out, count, cap, t);
+        unlock(idx, t);
+        if (hi <= node_t::high_key(m) || node_t::sibling(m) == kNoSibling) return count;
+        idx = node_t::sibling(m);
+      }
+      if (idx != kNone) flag(kErrCorrupt, t);
+      return count;
+    }
     if (idx == kNone || !to_version<CONC>(n, idx, ts, t)) return 0;
     uint32_t count = 0;
     for (uint32_t step = 0;; ++step) {
