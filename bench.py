@@ -225,6 +225,71 @@ def run_config2(args):
     return result
 
 
+def run_config3(args):
+    """Config 3: versioned tree, 40M-key build, then batches of 1M ops - 500K
+    erases of live keys and 500K finds (half hits among the keys live at the
+    snapshot, half misses) at the pre-batch snapshot - with epoch reclamation
+    after every batch. Reports throughput and the per-batch memory timeline
+    (live, peak, reclaimed and retired-but-unreclaimed node memory)."""
+    import torch
+
+    import mvgpubtree_amd as mv
+    from mvgpubtree_amd import workloads as wl
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    w = wl.config3(seed=3, n_initial=args.keys, num_batches=args.steps, ops_per_batch=args.ops)
+    ww = wl.config3(seed=33, n_initial=min(args.keys, 1_000_000), num_batches=1, ops_per_batch=args.ops // 10)
+    log(f"config3 workload generated in {time.perf_counter() - t0:.1f}s")
+
+    def run(tree, batches, timeline=None):
+        out = torch.empty(max(b.find_keys.size for b in batches), dtype=torch.int32, device=dev)
+        dev_b = [(dev_tensor(b.find_keys, dev), dev_tensor(b.erase_keys, dev)) for b in batches]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for fk, ek in dev_b:
+            s = tree.take_snapshot()
+            tree.find(fk, out[:fk.numel()], timestamp=s)
+            tree.erase(ek)
+            tree.release_snapshot(s)
+            tree.reclaim()
+            if timeline is not None:
+                st = tree.memory_stats()  # synchronous: between timed batches only
+                timeline.append({"live_MB": round(st.live_bytes / 1e6, 1), "peak_MB": round(st.peak_bytes / 1e6, 1),
+                                 "reclaimed_MB": round(st.reclaimed_nodes * st.node_bytes / 1e6, 1),
+                                 "retired_pending": st.retired_pending})
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    def make(n):
+        return mv.GpuVersionedBTree(pool_nodes=int(n / 4) + args.steps * args.ops + (1 << 20),
+                                    retire_capacity=1 << 23)
+
+    scratch = make(ww.initial_keys.size)
+    scratch.insert(dev_tensor(ww.initial_keys, dev), dev_tensor(ww.initial_values, dev))
+    run(scratch, ww.batches)
+    scratch.close()
+    tree = make(args.keys)
+    tree.insert(dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev))
+    torch.cuda.synchronize()
+    st0 = tree.memory_stats()
+    timeline = [{"live_MB": round(st0.live_bytes / 1e6, 1), "peak_MB": round(st0.peak_bytes / 1e6, 1),
+                 "reclaimed_MB": 0.0, "retired_pending": st0.retired_pending}]
+    # The timeline needs a statistics read after every batch (a host sync), so
+    # the throughput reported here includes those reads.
+    el = run(tree, w.batches, timeline)
+    ops = sum(b.num_ops for b in w.batches)
+    flags = tree.error_flags()
+    st = tree.memory_stats()
+    tree.close()
+    if flags:
+        raise RuntimeError(f"device error flags 0x{flags:x}")
+    return {"keys": args.keys, "batches": len(w.batches), "ops": ops,
+            "Mops_incl_reclaim_and_stats": round(ops / el / 1e6, 1),
+            "peak_live_MB": round(st.peak_bytes / 1e6, 1), "final_live_MB": round(st.live_bytes / 1e6, 1),
+            "reclaimed_MB": round(st.reclaimed_nodes * st.node_bytes / 1e6, 1), "timeline": timeline}
+
+
 def run_config4(args):
     """Batched form (config 4): 4096 batches, sizes log-uniform in [32, 65536],
     mix 60/20/10/10 find/insert/erase/range (8-128 keys), on a 40M-key tree.
@@ -432,8 +497,9 @@ def main():
     ap.add_argument("--ops", type=int, default=1_000_000)
     ap.add_argument("--node-width", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 4, 5),
-                    help="2: headline mixed insert/range batches; 4: batched form vs plain; "
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
+                    help="2: headline mixed insert/range batches; 3: erase/find batches with epoch "
+                         "reclamation and the memory timeline; 4: batched form vs plain; "
                          "5: served form vs plain per-request path")
     ap.add_argument("--served-keys", type=int, default=500_000_000, help="config 5: tree size")
     ap.add_argument("--qps", default="2000,8000,32000,128000", help="config 5: QPS steps")
@@ -453,6 +519,9 @@ def main():
         print("bench.py measures one GPU: the tree is a single-device structure here "
               "(multi-GPU key-range sharding is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.config == 3:
+        print(json.dumps({"config3": run_config3(args)}), flush=True)
+        return
     if args.config == 4:
         print(json.dumps({"config4": run_config4(args)}), flush=True)
         return

@@ -202,6 +202,15 @@ mvbt_status mvbt_concurrent_find_erase(mvbt_tree* tree, const uint32_t* find_key
                                        uint64_t num_finds, const uint32_t* erase_keys, uint64_t num_erasures,
                                        void* stream);
 
+/* Replaces plot_dot (gpu_versioned_blink_tree.hpp:1344-1377, gpu_blink_tree.hpp:986):
+ * Graphviz text of the latest tree, breadth-first from the root (record nodes
+ * with their pairs and high key; child, sibling and, for versioned leaves,
+ * version-chain edges), at most max_nodes nodes (0 = all). Writes at most
+ * capacity - 1 bytes plus a terminating NUL to buf (buf may be NULL) and the
+ * full length to *length, like snprintf; the caller renders it (the reference
+ * runs `dot` itself). Synchronous. */
+mvbt_status mvbt_export_dot(mvbt_tree* tree, char* buf, uint64_t capacity, uint64_t* length, uint32_t max_nodes);
+
 /* ---- Batched form: many batches of mixed operations in one launch ---------- */
 
 #define MVBT_OP_FIND 0u

@@ -50,7 +50,7 @@ EXPORTED_SYMBOLS = (
     "mvbt_memory_stats", "mvbt_error_flags", "mvbt_clear_error_flags", "mvbt_validate",
     "mvbt_export", "mvbt_node_width", "mvbt_kind_of", "mvbt_run_batches", "mvbt_set_worker_waves",
     "mvbt_memory_stats_async", "mvbt_device_view", "mvbt_bulk_build", "mvbt_concurrent_insert_range",
-    "mvbt_concurrent_find_erase",
+    "mvbt_concurrent_find_erase", "mvbt_export_dot",
 )
 
 
@@ -139,6 +139,7 @@ def load_library():
         "mvbt_bulk_build": ([vp, vp, vp, u64, u32, vp], i32),
         "mvbt_concurrent_insert_range": ([vp, vp, vp, u64, vp, vp, u64, vp, vp, u32, vp], i32),
         "mvbt_concurrent_find_erase": ([vp, vp, vp, u64, vp, u64, vp], i32),
+        "mvbt_export_dot": ([vp, ctypes.c_char_p, u64, u64p, u32], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -348,6 +349,14 @@ class _TreeBase:
         nk, h = ctypes.c_uint64(), ctypes.c_uint32()
         _check(load_library().mvbt_validate(self._h, ctypes.byref(nk), ctypes.byref(h)), "validate")
         return nk.value, h.value
+
+    def to_dot(self, max_nodes: int = 0) -> str:
+        """Graphviz text of the latest tree (reference plot_dot); render with `dot -Tpdf`."""
+        n = ctypes.c_uint64()
+        _check(load_library().mvbt_export_dot(self._h, None, 0, ctypes.byref(n), max_nodes), "export_dot")
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _check(load_library().mvbt_export_dot(self._h, buf, n.value + 1, ctypes.byref(n), max_nodes), "export_dot")
+        return buf.value.decode()
 
     def export(self):
         """Latest pairs in key order as a numpy uint64 array (key in the low word)."""

@@ -722,6 +722,73 @@ mvbt_status mvbt_export(mvbt_tree* t, mvbt_pair* out, uint64_t capacity, uint64_
   return MVBT_OK;
 }
 
+mvbt_status mvbt_export_dot(mvbt_tree* t, char* buf, uint64_t capacity, uint64_t* length, uint32_t max_nodes) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!length) return fail(MVBT_ERR_INVALID_ARG, "null length");
+  device_guard g(t->device);
+  host_walk w;
+  if (mvbt_status rc = copy_pool(t, w)) return rc;
+  if (mvbt_status rc = walk_tree(w, nullptr, nullptr, nullptr)) return rc;  // structure must be sound
+  const uint32_t nd = w.data_lanes(), nw = w.nw;
+  std::string dot = "digraph mvbt {\n  node [shape=record, height=.05, fontsize=9];\n";
+  auto pair_text = [](uint64_t p) {
+    return std::to_string(host_walk::key(p)) + ":" + std::to_string(host_walk::val(p));
+  };
+  auto emit = [&](uint32_t idx, bool copy) {
+    const uint64_t* n = w.at(idx);
+    const uint64_t m = n[nw - 1];
+    const uint32_t flags = (uint32_t)(m >> 32);
+    const bool leaf = (flags & kLeafBit) != 0;
+    std::string label = "<h>" + std::to_string(idx) + (copy ? " (copy)" : leaf ? " leaf" : "");
+    if (w.versioned && leaf) label += " @" + std::to_string((uint32_t)n[nw - 2]);
+    for (uint32_t l = 0; l < nd; ++l)
+      if (host_walk::key(n[l]) != kEmpty) label += "|<p" + std::to_string(l) + ">" + pair_text(n[l]);
+    label += "|hi " + ((uint32_t)m == kEmpty ? std::string("inf") : std::to_string((uint32_t)m));
+    dot += "  n" + std::to_string(idx) + " [label=\"" + label + "\"" + (copy ? ", style=dashed" : "") + "];\n";
+  };
+  // breadth-first over the latest tree, then each leaf's version chain
+  std::vector<uint32_t> level{0u};
+  uint64_t emitted = 0;
+  const uint64_t limit = max_nodes ? max_nodes : ~0ull;
+  while (!level.empty() && emitted < limit) {
+    std::vector<uint32_t> next;
+    for (uint32_t idx : level) {
+      if (emitted++ >= limit) break;
+      emit(idx, false);
+      const uint64_t* n = w.at(idx);
+      const uint64_t m = n[nw - 1];
+      const uint32_t flags = (uint32_t)(m >> 32);
+      const uint32_t sib = flags & kPtrMask;
+      if (sib != kNoSibling) dot += "  n" + std::to_string(idx) + " -> n" + std::to_string(sib) + " [style=dotted, constraint=false];\n";
+      if (!(flags & kLeafBit)) {
+        for (uint32_t l = 0; l < nd; ++l)
+          if (host_walk::key(n[l]) != kEmpty) {
+            const uint32_t c = host_walk::val(n[l]);
+            dot += "  n" + std::to_string(idx) + ":p" + std::to_string(l) + " -> n" + std::to_string(c) + ":h;\n";
+            next.push_back(c);
+          }
+      } else if (w.versioned) {
+        uint32_t older = (uint32_t)(n[nw - 2] >> 32), from = idx;
+        for (uint32_t hop = 0; older != kNone && older < w.nodes && hop < 64 && emitted < limit; ++hop, ++emitted) {
+          emit(older, true);
+          dot += "  n" + std::to_string(from) + " -> n" + std::to_string(older) + " [style=dashed, label=\"older\"];\n";
+          from = older;
+          older = (uint32_t)(w.at(older)[nw - 2] >> 32);
+        }
+      }
+    }
+    level.swap(next);
+  }
+  dot += "}\n";
+  *length = dot.size();
+  if (buf && capacity) {
+    const uint64_t k = std::min<uint64_t>(capacity - 1, dot.size());
+    std::memcpy(buf, dot.data(), k);
+    buf[k] = '\0';
+  }
+  return MVBT_OK;
+}
+
 mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
                              const uint64_t* batch_offsets, uint32_t num_batches, uint32_t* results,
                              mvbt_pair* pairs, uint64_t pair_slots, uint32_t L, uint32_t* snapshot_base,

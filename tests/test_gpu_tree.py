@@ -319,3 +319,30 @@ def test_device_api_user_kernels():
         r = subprocess.run([exe, n], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         assert "OK" in r.stdout
+
+
+@pytest.mark.parametrize("kind", ["plain", "versioned"])
+def test_export_dot(dev, kind):
+    """Graphviz text (reference plot_dot): every latest key appears once in a
+    leaf record, child and sibling edges are present, versioned leaves show
+    their version chains, and max_nodes bounds the output."""
+    rng = np.random.default_rng(8)
+    keys = wl.unique_keys(rng, 400)
+    cls = mv.GpuBlinkTree if kind == "plain" else mv.GpuVersionedBTree
+    tree = cls(pool_nodes=1 << 12)
+    tree.insert(to_dev(keys[:300], dev), to_dev(wl.value_of(keys[:300]), dev))
+    if kind == "versioned":
+        tree.take_snapshot()
+    tree.insert(to_dev(keys[300:], dev), to_dev(wl.value_of(keys[300:]), dev))
+    dot = tree.to_dot()
+    assert dot.startswith("digraph") and dot.rstrip().endswith("}")
+    leaf_lines = [l for l in dot.splitlines() if " leaf" in l and "label=" in l]
+    for k in keys[::17]:
+        field = f">{int(k)}:{int(wl.value_of(np.array([k]))[0])}|"  # record field "<pN>key:value|"
+        assert sum(field in l for l in leaf_lines) == 1, k
+    assert ":h;" in dot  # child edges
+    if kind == "versioned":
+        assert "older" in dot and "(copy)" in dot
+    small = tree.to_dot(max_nodes=3)
+    assert small.count("label=") <= 3
+    tree.close()
