@@ -256,6 +256,12 @@ mvbt_status mvbt_memory_stats_async(mvbt_tree* tree, mvbt_memstats* out, void* s
 /* Worker occupancy of the batched form in waves per CU (autotuner knob). */
 mvbt_status mvbt_set_worker_waves(mvbt_tree* tree, uint32_t waves_per_cu);
 
+/* Work unit of the batched form's worker (autotuner knob): 0 = one operation
+ * per tile; 1 = runs of consecutive operations per tile, sized from the batch
+ * sizes and handled with the chunk routing and batched leaf writes of the
+ * per-batch kernels (default). Same results either way. */
+mvbt_status mvbt_set_worker_mode(mvbt_tree* tree, uint32_t mode);
+
 /* ---- Device-side cooperative API ------------------------------------------ */
 
 /* Device pointers of a tree for user kernels (include/mvbt_device.hpp turns it

@@ -50,7 +50,7 @@ EXPORTED_SYMBOLS = (
     "mvbt_memory_stats", "mvbt_error_flags", "mvbt_clear_error_flags", "mvbt_validate",
     "mvbt_export", "mvbt_node_width", "mvbt_kind_of", "mvbt_run_batches", "mvbt_set_worker_waves",
     "mvbt_memory_stats_async", "mvbt_device_view", "mvbt_bulk_build", "mvbt_concurrent_insert_range",
-    "mvbt_concurrent_find_erase", "mvbt_export_dot",
+    "mvbt_concurrent_find_erase", "mvbt_export_dot", "mvbt_set_worker_mode",
 )
 
 
@@ -134,6 +134,7 @@ def load_library():
         "mvbt_kind_of": ([vp, u32p], i32),
         "mvbt_run_batches": ([vp, vp, u64, vp, u32, vp, vp, u64, u32, vp, vp, u32], i32),
         "mvbt_set_worker_waves": ([vp, u32], i32),
+        "mvbt_set_worker_mode": ([vp, u32], i32),
         "mvbt_memory_stats_async": ([vp, vp, vp], i32),
         "mvbt_device_view": ([vp, vp], i32),
         "mvbt_bulk_build": ([vp, vp, vp, u64, u32, vp], i32),
@@ -412,3 +413,7 @@ class GpuVersionedBTree(_TreeBase):
 
     def set_worker_waves(self, waves_per_cu: int):
         _check(load_library().mvbt_set_worker_waves(self._h, waves_per_cu), "set_worker_waves")
+
+    def set_worker_mode(self, mode: int):
+        """0: one operation per tile; 1: runs of operations per tile (default)."""
+        _check(load_library().mvbt_set_worker_mode(self._h, mode), "set_worker_mode")

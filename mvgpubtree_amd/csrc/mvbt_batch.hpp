@@ -286,4 +286,181 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
   tv.finish(ctx, t);
 }
 
+// 4'. The chunked worker. A tile claims a run of C consecutive operations
+// (lane j holds operation start + j) and works through it batch segment by
+// batch segment, each segment once its batch is open (the frontier rule of
+// the single-op worker). A segment uses the chunk engines of the per-batch
+// kernels: all its keys are routed together (descend_chunk), its inserts and
+// erases go through the batched leaf phase (write_chunk), its reads start
+// from their routed leaves, and its writes are counted with one atomic.
+// C follows the size of the batch the wave claimed from last: batch size /
+// tiles in the grid, clamped to [1, W], so small batches still spread one
+// operation per tile and large ones are taken W at a time. Claims stay in
+// operation order and each tile finishes its segments in order, so the
+// lowest unfinished operation is always held by a running tile whose earlier
+// segments are done: the queue drains as in the single-op worker.
+template <int NW>
+__global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, batch_job* job) {
+  const tree_view<NW, true> tv(tp);
+  using node_t = node<NW, true>;
+  tile<NW> t;
+  const unsigned long long nops = job->nops;
+  const uint32_t nb = job->nb, L = job->L, clock0 = job->clock0;
+  const op_record* ops = job->ops;
+  const unsigned long long* offsets = job->offsets;
+  uint32_t* results = job->results;
+  uint64_t* pairs = job->pairs;
+  const unsigned long long pair_slots = job->pair_slots;
+  const uint32_t lane = __lane_id(), r = t.rank();
+  const uint32_t my_tile = lane / NW;
+  const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  int32_t* const my_frontier = job->frontier + (wave % kFrontierCopies) * 32;
+  const unsigned long long tiles_total = (unsigned long long)gridDim.x * (blockDim.x / NW);
+  uint32_t backoff = 1;
+  write_ctx ctx = tv.begin_writes(t);
+  ctx.exact = 1;
+  int32_t seen = -1;
+  bool have = false;                       // tile holds an unfinished chunk (tile-uniform)
+  unsigned long long cstart = 0, seg = 0, end = 0;  // chunk start, next op, chunk end
+  uint32_t b = 0;                          // batch of op `seg`
+  uint32_t wave_b = 0;                     // batch of the wave's last claim (chunk sizing)
+  op_record op{};                          // lane j: operation cstart + j
+  uint32_t waits = 0;
+  for (;;) {
+    const uint64_t idle = __builtin_amdgcn_ballot_w64(!have && t.leader());
+    if (idle) {
+      const uint32_t n_idle = (uint32_t)__builtin_popcountll(idle);
+      const uint32_t first = (uint32_t)__builtin_ctzll(idle);
+      const unsigned long long bsz = offsets[wave_b + 1] - offsets[wave_b];
+      const unsigned long long cq = bsz / tiles_total;
+      const uint32_t C = cq < 1 ? 1u : (cq > (unsigned long long)NW ? (uint32_t)NW : (uint32_t)cq);
+      unsigned long long base = 0;
+      if (lane == first) base = atomic_add64(&job->next_op, (unsigned long long)n_idle * C);
+      base = ((unsigned long long)__builtin_amdgcn_readlane((int)(uint32_t)(base >> 32), (int)first) << 32) |
+             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, (int)first);
+      if (base < nops) wave_b = batch_after(offsets, nb, base, wave_b);
+      if (!have) {
+        const uint64_t below = (my_tile == 0) ? 0ull : (idle & ((1ull << (my_tile * NW)) - 1));
+        cstart = base + (unsigned long long)__builtin_popcountll(below) * C;
+        if (cstart >= nops) break;  // queue drained
+        end = cstart + C < nops ? cstart + C : nops;
+        seg = cstart;
+        if (cstart + r < end) op = ops[cstart + r];
+        b = batch_after(offsets, nb, seg, b);
+        have = true;
+      }
+    }
+    // the segment's batch must be open (all writes of earlier batches done)
+    const bool blocked = (int32_t)b - 1 > seen;
+    const uint64_t need = __builtin_amdgcn_ballot_w64(blocked && t.leader());
+    if (need) {
+      const uint32_t poller = (uint32_t)__builtin_ctzll(need);
+      int32_t f = 0;
+      if (lane == poller) f = __hip_atomic_load(my_frontier, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      f = __builtin_amdgcn_readlane(f, (int)poller);
+      if (f > seen) seen = f;
+    }
+    if ((int32_t)b - 1 > seen) {
+      if (++waits > (1u << 21)) {  // seconds: the frontier never moved
+        tv.flag(kErrCorrupt, t);
+        break;
+      }
+      if (__builtin_amdgcn_ballot_w64(!((int32_t)b - 1 > seen)) == 0) {
+        for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(8);
+        if (backoff < 16) backoff *= 2;
+      }
+      continue;
+    }
+    waits = 0;
+    backoff = 1;
+    // one segment: the chunk's operations of batch b
+    const unsigned long long seg_end = end < offsets[b + 1] ? end : offsets[b + 1];
+    const bool mine = cstart + r >= seg && cstart + r < seg_end;
+    const bool is_w = mine && (op.type == kOpInsert || op.type == kOpErase);
+    const bool is_rd = mine && !is_w;
+    uint32_t rkey = op.key;
+    if (op.type == kOpRange && rkey == 0) rkey = 1;
+    const bool route = mine && valid_key(rkey) && !(op.type == kOpRange && op.arg <= rkey);
+    uint32_t leaf = kNone, par = kNone, res = 0;
+    tv.template descend_chunk<true>(rkey, t.ballot(route), leaf, par, t);
+    // writes: batched leaf phase, then the keys it hands back one by one
+    ctx.cur = clock0 + b + 1;
+    const uint64_t qw = t.ballot(is_w);
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      uint64_t q = t.ballot(is_w && op.type == (w == 0 ? kOpInsert : kOpErase));
+      if (!q) continue;
+      ctx.want = (uint32_t)__builtin_popcountll(q);
+      q = (w == 0) ? tv.template write_chunk<0>(op.key, op.arg, q, leaf, res, ctx, t)
+                   : tv.template write_chunk<1>(op.key, 0u, q, leaf, res, ctx, t);
+      while (q) {
+        const uint32_t j = (uint32_t)lowest_bit(q);
+        ctx.want = (uint32_t)__builtin_popcountll(q);
+        q &= q - 1;
+        const uint32_t k = t.shfl(op.key, j);
+        const uint32_t v = (w == 0) ? tv.template write<0>(k, t.shfl(op.arg, j), ctx, t, t.shfl(leaf, j), t.shfl(par, j))
+                                    : tv.template write<1>(k, 0u, ctx, t, t.shfl(leaf, j), t.shfl(par, j));
+        if (r == j) res = v;
+      }
+    }
+    // reads at the batch's snapshot, from their routed leaves: finds together,
+    // ranges one by one
+    const uint64_t routed = t.ballot(route);
+    {
+      const uint64_t qf = t.ballot(is_rd && op.type == kOpFind);
+      if (qf) {
+        const uint32_t v = tv.find_chunk_conc(op.key, leaf, clock0 + b, qf & routed, t);
+        if ((qf >> r) & 1) res = v;
+      }
+    }
+    uint64_t qr = t.ballot(is_rd && op.type != kOpFind);
+    while (qr) {
+      const uint32_t j = (uint32_t)lowest_bit(qr);
+      qr &= qr - 1;
+      const uint32_t ty = t.shfl(op.type, j), k = t.shfl(op.key, j), a = t.shfl(op.arg, j);
+      const uint32_t lf = t.shfl(leaf, j);
+      const bool ok = (routed >> j) & 1;
+      uint32_t v;
+      if (ty == kOpFind) {
+        v = kEmpty;
+        if (ok) {
+          node_t n;
+          n.load(tv.at(lf), t);
+          v = tv.template find_in<true>(k, clock0 + b, n, lf, t);
+        }
+      } else {
+        v = 0;
+        const uint32_t sl = t.shfl(op.slot, j);
+        uint64_t* out = (pairs && sl < pair_slots) ? pairs + (size_t)sl * L : nullptr;
+        if (pairs && sl >= pair_slots) tv.flag(kErrCorrupt, t);
+        if (ok) {
+          node_t n;
+          n.load(tv.at(lf), t);
+          v = tv.template range_in<true>(k == 0 ? 1u : k, a, clock0 + b, n, lf, out, L, t);
+        }
+        if (v > L) tv.flag(kErrRangeOverflow, t);
+      }
+      if (r == j) res = v;
+    }
+    if (mine) results[cstart + r] = res;
+    // count the segment's writes: one atomic; the batch's last one opens the next
+    if (qw) {
+      order_release();  // the segment's stores complete before they are counted
+      if (t.leader()) {
+        const uint32_t cnt = (uint32_t)__builtin_popcountll(qw);
+        const uint32_t prev = atomic_add32(job->done + b, cnt);
+        if (prev + cnt == job->nwrites[b]) {
+          const int32_t adv = job->advance[b];
+          for (uint32_t c = 0; c < kFrontierCopies; ++c)
+            __hip_atomic_fetch_max(job->frontier + c * 32, adv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+    seg = seg_end;
+    if (seg >= end) have = false;
+    else b = batch_after(offsets, nb, seg, b);
+  }
+  tv.finish(ctx, t);
+}
+
 }  // namespace mvbt

@@ -67,6 +67,7 @@ struct mvbt_tree {
   int32_t* b_advance = nullptr;
   uint32_t b_capacity = 0;
   uint32_t worker_waves = 16;
+  uint32_t worker_mode = 1;  // 0: one operation per tile; 1: chunks (batch_chunk_worker_kernel)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   hipGraphNode_t graph_setup = nullptr;
@@ -184,6 +185,10 @@ template <int NW>
 void* worker_fn() {
   return reinterpret_cast<void*>(&batch_worker_kernel<NW>);
 }
+template <int NW>
+void* chunk_worker_fn() {
+  return reinterpret_cast<void*>(&batch_chunk_worker_kernel<NW>);
+}
 
 struct dispatch_table {
   mvbt_status (*init)(mvbt_tree*);
@@ -193,6 +198,7 @@ struct dispatch_table {
   mvbt_status (*insert)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, hipStream_t);
   mvbt_status (*erase)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, hipStream_t);
   void* (*worker)();
+  void* (*chunk_worker)();
   mvbt_status (*bulk)(mvbt_tree*, const uint32_t*, const uint32_t*, const bulk_plan&, hipStream_t);
   uint32_t data_lanes;
   using mixed_fn = mvbt_status (*)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, const uint32_t*,
@@ -205,7 +211,8 @@ template <int NW, bool VER>
 dispatch_table make_table() {
   using I = impl<NW, VER>;
   return dispatch_table{&I::init, &I::find, &I::range, &I::template write<0>, &I::template write<1>,
-                        VER ? &worker_fn<NW> : nullptr, &I::bulk, I::data_lanes(),
+                        VER ? &worker_fn<NW> : nullptr, VER ? &chunk_worker_fn<NW> : nullptr, &I::bulk,
+                        I::data_lanes(),
                         &I::template mixed<true>, &I::template mixed<false>};
 }
 
@@ -413,6 +420,11 @@ uint32_t worker_blocks(const mvbt_tree* t) {
   return (uint32_t)std::max<uint64_t>(b, 1);
 }
 
+void* worker_kernel(const mvbt_tree* t) {
+  const dispatch_table* tb = table_for(t->nw, true);
+  return t->worker_mode == 1 ? tb->chunk_worker() : tb->worker();
+}
+
 void destroy_graph(mvbt_tree* t) {
   if (t->graph_exec) (void)hipGraphExecDestroy(t->graph_exec);
   if (t->graph) (void)hipGraphDestroy(t->graph);
@@ -451,7 +463,7 @@ mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_frontier_kernel), t->num_cus, kBlock, job_args));
   tree_ptrs tp = t->ptrs();
   void* worker_args[] = {&tp, &job};
-  MVBT_HIP(add(table_for(t->nw, true)->worker(), worker_blocks(t), kBlock, worker_args));
+  MVBT_HIP(add(worker_kernel(t), worker_blocks(t), kBlock, worker_args));
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_finish_kernel), 1, 64, job_args));
   MVBT_HIP(hipGraphInstantiate(&t->graph_exec, t->graph, nullptr, nullptr, 0));
   return MVBT_OK;
@@ -836,7 +848,7 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
     tree_ptrs tp = t->ptrs();
     batch_job* job = t->job;
     void* wargs[] = {&tp, &job};
-    MVBT_HIP(hipLaunchKernel(table_for(t->nw, true)->worker(), dim3(worker_blocks(t)), dim3(kBlock),
+    MVBT_HIP(hipLaunchKernel(worker_kernel(t), dim3(worker_blocks(t)), dim3(kBlock),
                              wargs, 0, s));
     hipLaunchKernelGGL(batch_finish_kernel, dim3(1), dim3(64), 0, s, t->job);
   }
@@ -864,6 +876,19 @@ mvbt_status mvbt_set_worker_waves(mvbt_tree* t, uint32_t waves_per_cu) {
     (void)hipDeviceSynchronize();
     destroy_graph(t);
     t->worker_waves = waves_per_cu;
+  }
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_set_worker_mode(mvbt_tree* t, uint32_t mode) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (mode > 1) return fail(MVBT_ERR_INVALID_ARG, "worker mode is 0 (one operation per tile) or 1 (chunks)");
+  std::lock_guard<std::mutex> lk(t->mu);
+  if (t->worker_mode != mode) {
+    device_guard g(t->device);
+    (void)hipDeviceSynchronize();
+    destroy_graph(t);
+    t->worker_mode = mode;
   }
   return MVBT_OK;
 }

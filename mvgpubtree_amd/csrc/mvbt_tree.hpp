@@ -544,6 +544,93 @@ struct tree_view : tree_ptrs {
     return res;
   }
 
+  // Concurrent finds of a chunk at snapshot `ts` (versioned trees; writers may
+  // run), each key starting from its routed leaf (lane j: key, leaf). Pass-
+  // based like read_chunk: every pass loads one node for each unfinished key
+  // of a group. A live leaf whose stamp is visible gives a provisional answer
+  // that the next pass confirms by reading the leaf again and comparing the
+  // version words (the validation of to_version<true>); a leaf whose stamp is
+  // newer than ts hands over to its version chain, whose copies are immutable
+  // once published. Returns the value or kEmpty.
+  MVBT_DEV uint32_t find_chunk_conc(uint32_t key, uint32_t leaf, uint32_t ts, uint64_t todo,
+                                    const tile_t& t) const {
+    enum : uint32_t { kLive = 0, kCheck = 1, kChain = 2 };
+    const uint32_t r = t.rank();
+    uint32_t cur = leaf, phase = kLive, res = kEmpty, seen_ver_lo = 0, seen_ver_hi = 0, prov = kEmpty;
+    uint32_t err = 0;
+    todo &= t.ballot(valid_key(key) && leaf < cap_);
+    for (int base = 0; base < NW; base += kReadGroup) {
+      uint64_t active = (todo >> base) & ((kReadGroup == 64) ? ~0ull : ((1ull << kReadGroup) - 1));
+      for (uint32_t pass = 0; active != 0; ++pass) {
+        if (pass >= kMaxReadSteps) {
+          err |= kErrCorrupt;
+          break;
+        }
+        uint64_t p[kReadGroup];
+#pragma unroll
+        for (int j = 0; j < kReadGroup; ++j) {
+          const uint32_t idx = t.shfl(cur, (uint32_t)(base + j));
+          if ((active >> j) & 1) p[j] = ld_coherent(at(idx) + r);
+        }
+#pragma unroll
+        for (int j = 0; j < kReadGroup; ++j) {
+          if (!((active >> j) & 1)) continue;
+          const uint32_t lj = (uint32_t)(base + j);
+          node_t n;
+          n.p = p[j];
+          const uint64_t ver = n.version(t);
+          uint32_t ph = t.shfl(phase, lj);
+          bool done = false;
+          uint32_t next = t.shfl(cur, lj), out = kEmpty, keep_prov = kEmpty;
+          if (ph == kCheck) {
+            const uint64_t was = pack(t.shfl(seen_ver_lo, lj), t.shfl(seen_ver_hi, lj));
+            if (ver == was) {
+              out = t.shfl(prov, lj);
+              done = true;
+            } else {
+              ph = kLive;  // the leaf moved on: treat this read as a fresh one
+            }
+          }
+          if (!done && (ph == kLive || ph == kChain)) {
+            if (node_t::stamp(ver) > ts) {
+              next = node_t::older(ver);
+              ph = kChain;
+              if (next == kNone) {
+                err |= kErrVersionGap;
+                done = true;
+              } else if (next >= cap_) {
+                err |= kErrCorrupt;
+                done = true;
+              }
+            } else if (ph == kChain) {
+              out = n.lookup(t.shfl(key, lj), t);
+              done = true;
+            } else {  // visible live leaf: answer provisionally, confirm next pass
+              keep_prov = n.lookup(t.shfl(key, lj), t);
+              ph = kCheck;
+            }
+          }
+          if (r == lj) {
+            if (done) {
+              res = out;
+            } else {
+              cur = next;
+              phase = ph;
+              if (ph == kCheck) {
+                prov = keep_prov;
+                seen_ver_lo = lo32(ver);
+                seen_ver_hi = hi32(ver);
+              }
+            }
+          }
+          if (done) active &= ~(1ull << j);
+        }
+      }
+    }
+    if (err) flag(err, t);
+    return res;
+  }
+
   // ---- find -----------------------------------------------------------------
   template <bool CONC>
   MVBT_DEV uint32_t find(uint32_t key, uint32_t ts, const tile_t& t) const {

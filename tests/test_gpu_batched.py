@@ -28,11 +28,14 @@ def build_tree(w, dev, nw=16):
     return tree
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("nw", [16, 32, 64])
-def test_batched_form_matches_oracle(dev, nw, use_graph):
+def test_batched_form_matches_oracle(dev, nw, use_graph, mode):
+    """mode 0: one operation per tile; mode 1: runs of operations per tile."""
     w = small_config4(seed=40 + nw)
     tree = build_tree(w, dev, nw)
+    tree.set_worker_mode(mode)
     r = drivers.BatchedRunner(w.batches, w.range_capacity, dev, seed=nw)
     s0 = tree.take_snapshot()
     tree.release_snapshot(s0)
@@ -45,6 +48,29 @@ def test_batched_form_matches_oracle(dev, nw, use_graph):
     assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
     # clock advanced by one snapshot per batch
     assert tree.take_snapshot() == s0 + 1 + len(w.batches)
+    exp = tree.export()
+    assert np.array_equal(exp, final.keys.astype(np.uint64) | (final.values.astype(np.uint64) << np.uint64(32)))
+    assert tree.error_flags() == 0
+    tree.close()
+
+
+@pytest.mark.parametrize("waves", [1, 2])
+def test_chunked_worker_multi_op_chunks(dev, waves):
+    """Few worker waves and large batches, so tiles claim runs of up to 16
+    operations that span batch boundaries (batch size / tiles in the grid)."""
+    w = wl.config4(seed=77 + waves, n_initial=300_000, num_batches=40, min_batch=32, max_batch=65536,
+                   range_capacity=256)
+    tree = mv.GpuVersionedBTree(pool_nodes=1 << 20, node_width=16)
+    tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
+    tree.set_worker_mode(1)
+    tree.set_worker_waves(waves)
+    r = drivers.BatchedRunner(w.batches, w.range_capacity, dev, seed=waves)
+    res = to_np(r.run(tree, use_graph=True))
+    want, wpairs, final = oracle.batch_expected(w.initial_keys, w.initial_values, r.ops_host,
+                                                r.offsets_host, w.range_capacity, r.slots)
+    bad = np.nonzero(res != want)[0]
+    assert bad.size == 0, (bad[:10], r.ops_host[bad[:10]], res[bad[:10]], want[bad[:10]])
+    assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
     exp = tree.export()
     assert np.array_equal(exp, final.keys.astype(np.uint64) | (final.values.astype(np.uint64) << np.uint64(32)))
     assert tree.error_flags() == 0
