@@ -309,15 +309,18 @@ def run_config4(args):
     L = w.range_capacity
     waves = tuple(int(x) for x in args.waves.split(","))
     widths = tuple(int(x) for x in args.widths.split(","))
-    if len(widths) * len(waves) > 1:
-        tw = wl.config4(seed=44, n_initial=args.keys, num_batches=2 * len(waves) * args.tune_batches)
-        groups = [tw.batches[i::2 * len(waves)] for i in range(2 * len(waves))]
+    modes = tuple(int(x) for x in args.modes.split(","))
+    ncfg = len(waves) * len(modes)
+    if len(widths) * ncfg > 1:
+        tw = wl.config4(seed=44, n_initial=args.keys, num_batches=2 * ncfg * args.tune_batches)
+        groups = [tw.batches[i::2 * ncfg] for i in range(2 * ncfg)]
         best, allr = autotune.tune_batched(tw.initial_keys, tw.initial_values, groups, L, dev,
-                                           node_widths=widths, waves=waves, log=log)
+                                           node_widths=widths, waves=waves, log=log, modes=modes)
         del tw
     else:
-        best, allr = autotune.TuneResult(widths[0], waves[0], 0.0, 0.0), []
-    log(f"autotune best: node width {best.node_width}, {best.waves_per_cu} waves/CU")
+        best, allr = autotune.TuneResult(widths[0], waves[0], 0.0, 0.0, modes[0]), []
+    log(f"autotune best: node width {best.node_width}, worker mode {best.worker_mode}, "
+        f"{best.waves_per_cu} waves/CU")
     n_ops = sum(b.num_ops for b in w.batches)
     pool = int(args.keys / 4 * (16 / best.node_width) + 2 * n_ops) + (1 << 20)
     dk, dv = dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev)
@@ -326,6 +329,7 @@ def run_config4(args):
         t = mv.GpuVersionedBTree(pool_nodes=pool, node_width=best.node_width, retire_capacity=1 << 24)
         t.insert(dk, dv)
         t.set_worker_waves(best.waves_per_cu)
+        t.set_worker_mode(best.worker_mode)
         return t
 
     # batched form: warm the graph on a scratch run, then one timed launch
@@ -334,6 +338,7 @@ def run_config4(args):
     scratch = mv.GpuVersionedBTree(pool_nodes=1 << 21, node_width=best.node_width)
     scratch.insert(dev_tensor(wb.initial_keys, dev), dev_tensor(wb.initial_values, dev))
     scratch.set_worker_waves(best.waves_per_cu)
+    scratch.set_worker_mode(best.worker_mode)
     drivers.BatchedRunner(wb.batches, L, dev).run(scratch, use_graph=True)
     drivers.PerBatchRunner(wb.batches[:8], L, dev).run(scratch)
     torch.cuda.synchronize()
@@ -367,8 +372,9 @@ def run_config4(args):
         "speedup": round(plain_s / batched_s, 2),
         "batched_ms": round(batched_s * 1e3, 2), "plain_ms": round(plain_s * 1e3, 2),
         "ops": n_ops, "batches": len(w.batches), "failed_writes": fails,
-        "tuned": {"node_width": best.node_width, "waves_per_cu": best.waves_per_cu},
-        "tuning": [{"nw": r.node_width, "waves": r.waves_per_cu, "Mops": round(r.mops, 1)} for r in allr],
+        "tuned": {"node_width": best.node_width, "worker_mode": best.worker_mode, "waves_per_cu": best.waves_per_cu},
+        "tuning": [{"nw": r.node_width, "mode": r.worker_mode, "waves": r.waves_per_cu, "Mops": round(r.mops, 1)}
+                   for r in allr],
     }
 
 
@@ -412,6 +418,10 @@ def run_config5(args):
     vals = init ^ torch.tensor(wl.VALUE_XOR - 2**32, dtype=torch.int32, device=dev)
     pool = int(n / 6) + (8 << 20)
     tree = mv.GpuVersionedBTree(pool_nodes=pool, retire_capacity=1 << 24)
+    # Requests are small (mean about 1.7K ops), so the chunked worker would take
+    # runs of one op anyway; the one-op worker has the smaller register
+    # footprint and measured faster for the served steps.
+    tree.set_worker_mode(0)
     torch.cuda.synchronize()
     tb = time.perf_counter()
     tree.insert(init, vals)
@@ -507,6 +517,7 @@ def main():
     ap.add_argument("--step-ops", type=int, default=1 << 20, help="config 5: max ops per served step")
     ap.add_argument("--batches", type=int, default=4096, help="config 4: number of batches")
     ap.add_argument("--tune-batches", type=int, default=48, help="config 4: batches per tuning group")
+    ap.add_argument("--modes", default="0,1", help="config 4: worker modes to tune (0 one op per tile, 1 runs)")
     ap.add_argument("--widths", default="16,32,64", help="config 4: node widths to tune")
     ap.add_argument("--waves", default="8,16,24,32", help="config 4: waves/CU to tune")
     ap.add_argument("--profile-copy", action="store_true",

@@ -1,7 +1,8 @@
 """Autotuner for the batched form.
 
 Sweeps node width (16, 32, 64 pairs; the tile width equals the node width,
-one pair per lane) and worker occupancy (waves per CU), times the batched
+one pair per lane), worker mode (one operation per tile, or runs of
+operations per tile) and worker occupancy (waves per CU), times the batched
 form on a sample of the target workload against a tree of the target size,
 and returns the fastest configuration. Results can be cached in a JSON file
 keyed by device name and workload shape.
@@ -24,6 +25,7 @@ class TuneResult:
     waves_per_cu: int
     ms: float
     mops: float
+    worker_mode: int = 1
 
 
 def _device_name() -> str:
@@ -33,21 +35,23 @@ def _device_name() -> str:
 
 def tune_batched(initial_keys, initial_values, groups, L: int, dev,
                  node_widths=(16, 32, 64), waves=(8, 16, 24, 32), cache: str | None = None,
-                 log=None):
+                 log=None, modes=(1,)):
     """Returns (best TuneResult, all results).
 
-    `groups` holds 2 * len(waves) disjoint lists of batches cut from one
-    workload (fresh insert keys and distinct erase keys throughout): group 2i
-    warms waves[i] up (graph build for that grid, code load), group 2i + 1 is
-    timed at waves[i]. Every node width gets its own tree built from the full
-    initial key set, so each configuration starts from the same tree."""
+    `groups` holds 2 * len(waves) * len(modes) disjoint lists of batches cut
+    from one workload (fresh insert keys and distinct erase keys throughout):
+    for configuration i (a mode and a wave count), group 2i warms it up (graph
+    build for that grid, code load) and group 2i + 1 is timed. Every node width
+    gets its own tree built from the full initial key set, so each
+    configuration starts from the same tree."""
     import torch
 
     import mvgpubtree_amd as mv
 
-    if len(groups) != 2 * len(waves):
-        raise ValueError("need 2 * len(waves) batch groups")
-    key = f"{_device_name()}|{initial_keys.size}|{[len(g) for g in groups]}|{L}"
+    combos = [(m, wv) for m in modes for wv in waves]
+    if len(groups) != 2 * len(combos):
+        raise ValueError("need 2 * len(waves) * len(modes) batch groups")
+    key = f"{_device_name()}|{initial_keys.size}|{[len(g) for g in groups]}|{L}|{list(modes)}"
     if cache and os.path.exists(cache):
         try:
             data = json.load(open(cache))
@@ -65,7 +69,8 @@ def tune_batched(initial_keys, initial_values, groups, L: int, dev,
         pool = int(initial_keys.size / 4 * (16 / nw) + 2 * total_ops) + (1 << 20)
         tree = mv.GpuVersionedBTree(pool_nodes=pool, node_width=nw, retire_capacity=1 << 24)
         tree.insert(dk, dv)
-        for i, wv in enumerate(waves):
+        for i, (mode, wv) in enumerate(combos):
+            tree.set_worker_mode(mode)
             tree.set_worker_waves(wv)
             runners[2 * i].run(tree, use_graph=True)
             tree.reclaim()
@@ -76,10 +81,10 @@ def tune_batched(initial_keys, initial_values, groups, L: int, dev,
             torch.cuda.synchronize()
             ms = (time.perf_counter() - t0) * 1e3
             tree.reclaim()
-            r = TuneResult(nw, wv, ms, runner.nops / ms / 1e3)
+            r = TuneResult(nw, wv, ms, runner.nops / ms / 1e3, mode)
             results.append(r)
             if log:
-                log(f"autotune nw={nw} waves/CU={wv}: {ms:.2f} ms ({r.mops:.1f} M ops/s)")
+                log(f"autotune nw={nw} mode={mode} waves/CU={wv}: {ms:.2f} ms ({r.mops:.1f} M ops/s)")
         if tree.error_flags():
             raise RuntimeError(f"autotune: device error flags 0x{tree.error_flags():x}")
         tree.close()

@@ -129,3 +129,23 @@ def test_batched_snapshot_reads_older_than_the_run(dev):
     assert np.array_equal(got, oracle.SortedState(w.initial_keys, w.initial_values).find(q))
     assert tree.error_flags() == 0
     tree.close()
+
+
+def test_autotuner_sweeps_and_caches(dev, tmp_path):
+    """The autotuner times every (node width, worker mode, waves/CU) and picks
+    the fastest; a second call with the same key reads the cache."""
+    from mvgpubtree_amd import autotune
+    w = wl.config4(seed=5, n_initial=100_000, num_batches=32, min_batch=32, max_batch=4096, range_capacity=256)
+    waves, modes = (4, 8), (0, 1)
+    n = 2 * len(waves) * len(modes)
+    groups = [w.batches[i::n] for i in range(n)]
+    cache = str(tmp_path / "tune.json")
+    best, allr = autotune.tune_batched(w.initial_keys, w.initial_values, groups, w.range_capacity, dev,
+                                       node_widths=(16, 32), waves=waves, modes=modes, cache=cache)
+    assert len(allr) == 2 * len(waves) * len(modes)
+    assert {(r.node_width, r.worker_mode, r.waves_per_cu) for r in allr} == \
+        {(nw, m, wv) for nw in (16, 32) for m in modes for wv in waves}
+    assert best.mops == max(r.mops for r in allr)
+    again, _ = autotune.tune_batched(w.initial_keys, w.initial_values, groups, w.range_capacity, dev,
+                                     node_widths=(16, 32), waves=waves, modes=modes, cache=cache)
+    assert again == best
