@@ -734,7 +734,7 @@ struct tree_view : tree_ptrs {
   }
 
   // ---- writers ----------------------------------------------------------------
-  enum step : uint32_t { kAtRoot = 0, kAtParent = 1, kOutOfMemory = 2 };
+  enum step : uint32_t { kAtRoot = 0, kAtParent = 1, kOutOfMemory = 2, kInserted = 3 };
   static constexpr uint32_t kWriteSame = 0, kWriteNew = 1, kWriteFailed = kEmpty;
 
   // Splits the full root in place: its pairs move to two new children and the
@@ -850,8 +850,12 @@ struct tree_view : tree_ptrs {
 
   // Splits locked full leaf `idx` (loaded in `n`) under locked-and-checked
   // parent; always unlocks `idx`.
-  MVBT_DEV uint32_t split_leaf(node_t& n, uint32_t idx, uint32_t parent, write_ctx& c,
-                               const tile_t& t) const {
+  // When the split succeeds, `key` (absent from the full leaf) goes into the
+  // half that covers it before either half is stored, and kInserted is
+  // returned: the insert that needed the split costs no second descent, lock
+  // and store round.
+  MVBT_DEV uint32_t split_leaf(node_t& n, uint32_t idx, uint32_t parent, write_ctx& c, const tile_t& t,
+                               uint32_t key, uint32_t value) const {
     if (parent == kNone) {
       unlock(idx, t);
       return kAtRoot;
@@ -878,9 +882,11 @@ struct tree_view : tree_ptrs {
     uint32_t pivot = n.split_pivot(t);
     uint64_t ver = VER ? n.version(t) : 0ull;
     node_t yn = n.upper_half(m, ver, t);
+    n.keep_lower_half(pivot, y, t);
+    if (key >= pivot) yn.insert(key, value, t);
+    else n.insert(key, value, t);
     yn.store(at(y), t);
     order_release();
-    n.keep_lower_half(pivot, y, t);
     if constexpr (VER) {
       if (t.rank() != node_t::kVerLane) st_coherent(at(idx) + t.rank(), n.p);
     } else {
@@ -891,7 +897,7 @@ struct tree_view : tree_ptrs {
     pn.store_body(at(parent), t);
     unlock(parent, t);
     unlock(idx, t);
-    return kAtParent;
+    return kInserted;
   }
 
   // Shared writer loop. OP = 0 insert/update, 1 erase. Returns kWriteNew
@@ -975,8 +981,12 @@ struct tree_view : tree_ptrs {
           return kWriteSame;
         }
       } else if (n.is_full(t) && !n.contains(key, t)) {
-        uint32_t s = split_leaf(n, idx, parent, c, t);
+        uint32_t s = split_leaf(n, idx, parent, c, t, key, value);
         if (s == kOutOfMemory) return kWriteFailed;
+        if (s == kInserted) {
+          c.backoff = c.backoff > 1 ? c.backoff >> 1 : 1u;
+          return kWriteNew;
+        }
         idx = (s == kAtParent && parent != kNone) ? parent : root_;
         parent = kNone;
         continue;
