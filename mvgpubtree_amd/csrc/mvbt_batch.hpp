@@ -386,7 +386,6 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
     // writes: batched leaf phase, then the keys it hands back one by one
     ctx.cur = clock0 + b + 1;
     const uint64_t qw = t.ballot(is_w);
-#pragma unroll
     for (int w = 0; w < 2; ++w) {
       uint64_t q = t.ballot(is_w && op.type == (w == 0 ? kOpInsert : kOpErase));
       if (!q) continue;
