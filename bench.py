@@ -60,15 +60,71 @@ def cpu_baseline_config2(w, batch):
     m.build_sorted(ks, (ks ^ np.uint32(0x9E3779B9)).astype(np.uint32))
     t0 = time.perf_counter()
     s = m.take_snapshot()
-    m.range_query(batch.range_lo, batch.range_hi, w.range_capacity, s)
+    total1 = int(m.range_query(batch.range_lo, batch.range_hi, w.range_capacity, s)[1].sum())
     m.insert(batch.insert_keys, batch.insert_values)
     dt = time.perf_counter() - t0
     ops = batch.num_ops
     m.close()
-    return {"value": round(ops / dt / 1e6, 3), "unit": "M ops/s", "cores": 1, "kind": "port",
-            "sample": f"1 batch ({ops} ops: {batch.range_lo.size} range-32 + "
-                      f"{batch.insert_keys.size} inserts) on the {w.initial_keys.size}-key map, "
-                      f"{dt:.2f} s on one host core"}
+    # the same batch over key-range shards, one host thread each (the box's CPU
+    # share for one GPU is 16 cores)
+    P = max(1, min(16, os.cpu_count() or 1))
+    vP, dtP, totalP = cpu_baseline_config2_sharded(w, batch, P)
+    if totalP != total1:
+        raise RuntimeError(f"sharded CPU baseline disagrees with one map: {totalP} vs {total1} range pairs")
+    return {"value": round(vP, 3), "unit": "M ops/s", "cores": P, "kind": "port",
+            "sample": f"1 batch ({ops} ops: {batch.range_lo.size} range-32 + {batch.insert_keys.size} "
+                      f"inserts) on the {w.initial_keys.size}-key map: {P} threads over key-range shards "
+                      f"{dtP:.2f} s; one thread on one map {ops / dt / 1e6:.3f} M ops/s ({dt:.2f} s)"}
+
+
+def cpu_baseline_config2_sharded(w, batch, threads: int):
+    """The same batch on `threads` host threads: the key space is split into
+    key-range shards (splitters from the initial keys), each shard is its own
+    oracle map served by one thread, and a range query is clipped to every
+    shard it overlaps (its counts add up). The oracle's C calls run outside
+    the interpreter lock, so the threads run in parallel."""
+    import threading
+
+    import oracle
+    ks = np.sort(w.initial_keys)
+    vs = (ks ^ np.uint32(0x9E3779B9)).astype(np.uint32)
+    P = threads
+    cut = np.concatenate([[0], ks[(np.arange(1, P) * ks.size) // P].astype(np.uint64), [1 << 32]])
+    maps = [oracle.MVMap() for _ in range(P)]
+    lo64, hi64 = batch.range_lo.astype(np.uint64), batch.range_hi.astype(np.uint64)
+    work = []
+    for p in range(P):
+        a, b = cut[p], cut[p + 1]
+        sel = (ks >= a) & (ks < b)
+        ins = (batch.insert_keys >= a) & (batch.insert_keys < b)
+        hit = (lo64 < b) & (hi64 > a)
+        work.append((ks[sel], vs[sel], batch.insert_keys[ins], batch.insert_values[ins],
+                     np.maximum(lo64[hit], a).astype(np.uint32),
+                     np.minimum(hi64[hit], b).astype(np.uint32) if b < (1 << 32)
+                     else hi64[hit].astype(np.uint32)))
+    builders = [threading.Thread(target=maps[p].build_sorted, args=(work[p][0], work[p][1])) for p in range(P)]
+    for th in builders:
+        th.start()
+    for th in builders:
+        th.join()
+    counts = [None] * P
+
+    def serve(p):
+        _, _, ik, iv, lo, hi = work[p]
+        s = maps[p].take_snapshot()
+        counts[p] = maps[p].range_query(lo, hi, w.range_capacity, s)[1]
+        maps[p].insert(ik, iv)
+
+    runners = [threading.Thread(target=serve, args=(p,)) for p in range(P)]
+    t0 = time.perf_counter()
+    for th in runners:
+        th.start()
+    for th in runners:
+        th.join()
+    dt = time.perf_counter() - t0
+    for m in maps:
+        m.close()
+    return batch.num_ops / dt / 1e6, dt, int(sum(int(c.sum()) for c in counts))
 
 
 def run_config2(args):
