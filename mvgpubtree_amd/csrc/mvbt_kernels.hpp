@@ -49,19 +49,23 @@ __global__ void __launch_bounds__(256) find_kernel(tree_view<NW, VER> tv, const 
                                                    uint32_t* values, uint64_t n, uint32_t ts) {
   tile<NW> t;
   chunking<NW> c;
+  __shared__ route_cache<NW> rc;
+  tv.build_route_cache(rc);
   for (uint64_t base = c.first; base < n; base += c.stride) {
     uint64_t i = base + t.rank();
     bool active = i < n;
     uint32_t key = active ? keys[i] : 0u;
     uint32_t result = kEmpty;
     uint64_t q = t.ballot(active);
-    // Key by key: a find is one root-to-leaf walk whose upper levels stay in
+    uint32_t start = kNone, unused;  // each lane routes its key through the cached levels
+    if (active && valid_key(key)) start = tv.route_lane(rc, key, unused);
+    // Key by key: a find is one walk to a leaf whose upper levels stay in
     // the L1, and the small register footprint keeps 8 waves per SIMD in
     // flight; the pass-based engine (read_chunk) measured slower here.
     while (q) {
       uint32_t j = (uint32_t)lowest_bit(q);
       q &= q - 1;
-      uint32_t r = tv.template find<CONC>(t.shfl(key, j), ts, t);
+      uint32_t r = tv.template find<CONC>(t.shfl(key, j), ts, t, t.shfl(start, j));
       if (t.rank() == j) result = r;
     }
     if (active) values[i] = result;
@@ -74,6 +78,8 @@ __global__ void __launch_bounds__(256)
                  uint64_t* out, uint32_t* counts, uint32_t L, uint64_t n, uint32_t ts) {
   tile<NW> t;
   chunking<NW> c;
+  __shared__ route_cache<NW> rc;
+  tv.build_route_cache(rc);
   for (uint64_t base = c.first; base < n; base += c.stride) {
     uint64_t i = base + t.rank();
     bool active = i < n;
@@ -84,13 +90,15 @@ __global__ void __launch_bounds__(256)
     if constexpr (!CONC) {
       // no writer runs: every query of the chunk advances one node per pass
       result = tv.template read_chunk<true>(lo == 0 ? 1u : lo, hi, ts, out ? out + base * (uint64_t)L : nullptr,
-                                            L, q, t);
+                                            L, q, t, &rc);
     } else {
+      uint32_t start = kNone, unused;
+      if (active && valid_key(lo == 0 ? 1u : lo)) start = tv.route_lane(rc, lo == 0 ? 1u : lo, unused);
       while (q) {
         uint32_t j = (uint32_t)lowest_bit(q);
         q &= q - 1;
         uint64_t* slot = out ? out + (base + j) * (uint64_t)L : nullptr;
-        uint32_t cnt = tv.template range<CONC>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t);
+        uint32_t cnt = tv.template range<CONC>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t, t.shfl(start, j));
         if (t.rank() == j) result = cnt;
       }
     }
@@ -104,6 +112,8 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
                                                     const uint32_t* values, uint64_t n) {
   tile<NW> t;
   chunking<NW> c;
+  // No route cache here: at one chunk per tile the block-wide copy of the
+  // top levels delays every tile by more than the levels it saves.
   if (c.first >= n) return;
   write_ctx ctx = tv.begin_writes(t);
   ctx.exact = 1;  // ctx.want below is the chunk's exact remaining count

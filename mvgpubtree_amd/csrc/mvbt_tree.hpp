@@ -104,6 +104,29 @@ struct tree_ptrs {
   uint32_t cap_;           // pool capacity: every followed index must be below it
 };
 
+// The top levels of the tree, copied into LDS by each workgroup of a read
+// kernel when it starts (tree_view::build_route_cache) and used for routing
+// only (tree_view::route_lane): a find or range query then reads from global
+// memory only the levels below them. Levels are cached breadth-first, whole,
+// while they hold internal nodes only and fit kBytes; the children of a
+// cached node occupy consecutive slots in entry order. An image may go stale
+// while the kernel runs (a concurrent writer's splits); it then routes to a
+// node left of the right one, which the B-link right moves of the first
+// uncached level repair. At 40M keys and 16-pair nodes the cache holds the
+// root and the two levels below it (about 45 nodes).
+
+template <int NW>
+struct route_cache {
+  static constexpr uint32_t kBytes = 16384;
+  static constexpr uint32_t kMaxNodes = kBytes / (NW * 8);
+  static constexpr uint16_t kNoChild = 0xFFFF;
+  uint64_t nodes[kMaxNodes * NW];
+  uint32_t index[kMaxNodes];        // global index of the node in slot s
+  uint16_t first_child[kMaxNodes];  // slot of its first child, or kNoChild
+  uint32_t count;                   // slots routed through (internal node images)
+  uint32_t stop;                    // build: the last level has been reached
+};
+
 template <int NW, bool VER>
 struct tree_view : tree_ptrs {
   using node_t = node<NW, VER>;
@@ -344,8 +367,13 @@ struct tree_view : tree_ptrs {
   // (routing only: a stale image routes too far left and the right moves
   // repair it), and a node found to be a leaf is read again coherently.
   template <bool CONC, bool CACHED_ROUTE = false>
-  MVBT_DEV uint32_t descend(uint32_t key, node_t& n, const tile_t& t, uint32_t* parent = nullptr) const {
+  MVBT_DEV uint32_t descend(uint32_t key, node_t& n, const tile_t& t, uint32_t* parent = nullptr,
+                            uint32_t start = kNone, uint32_t start_parent = kNone) const {
     uint32_t idx = root_, par = kNone;
+    if (start < cap_) {
+      idx = start;
+      par = start_parent;
+    }
     for (uint32_t step = 0; step < kMaxReadSteps; ++step) {
       if constexpr (CONC && CACHED_ROUTE) {
         n.load_weak(at(idx), t);
@@ -452,13 +480,17 @@ struct tree_view : tree_ptrs {
   static constexpr int kReadGroup = NW < 8 ? NW : 8;
   template <bool RANGE>
   MVBT_DEV uint32_t read_chunk(uint32_t lo, uint32_t hi, uint32_t ts, uint64_t* out, uint32_t cap,
-                               uint64_t todo, const tile_t& t) const {
+                               uint64_t todo, const tile_t& t, const route_cache<NW>* rc = nullptr) const {
     enum : uint32_t { kRoute = 0, kArrive = 1, kChain = 2 };
     const uint32_t r = t.rank();
     uint32_t cur = root_, phase = kRoute, cnt = 0;
     uint32_t res = RANGE ? 0u : kEmpty;
     uint32_t err = 0;
     todo &= t.ballot(valid_key(lo) && (!RANGE || hi > lo));
+    if (rc && ((todo >> r) & 1)) {  // the cached top levels
+      uint32_t unused;
+      cur = route_lane(*rc, lo, unused);
+    }
     for (int base = 0; base < NW; base += kReadGroup) {
       uint64_t active = (todo >> base) & ((kReadGroup == 64) ? ~0ull : ((1ull << kReadGroup) - 1));
       for (uint32_t pass = 0; active != 0; ++pass) {
@@ -631,12 +663,124 @@ struct tree_view : tree_ptrs {
     return res;
   }
 
+  // Block-wide (every thread of the workgroup calls it, before any early
+  // return; blockDim.x >= 64): copies the top levels into `rc`.
+  MVBT_DEV void build_route_cache(route_cache<NW>& rc) const {
+    using rc_t = route_cache<NW>;
+    const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
+    if (tid == 0) {
+      rc.index[0] = root_;
+      rc.count = 1;
+      rc.stop = 0;
+    }
+    __syncthreads();
+    uint32_t b = 0, e = 1;
+    // every level that does not stop the build adds at least one slot, so
+    // the build stops (rc.stop) within kMaxNodes levels
+    for (uint32_t level = 0; level < rc_t::kMaxNodes; ++level) {
+      for (uint32_t w = tid; w < (e - b) * NW; w += nthreads) {
+        const uint32_t s = b + w / NW;
+        rc.nodes[s * NW + w % NW] = ld_coherent(at(rc.index[s]) + w % NW);
+      }
+      __syncthreads();
+      if (tid < 64) {  // wave 0: child counts, prefix sum, child slots
+        const uint32_t lane = tid;
+        uint32_t next = e;
+        bool all_internal = true;
+        for (uint32_t s0 = b; s0 < e; s0 += 64) {
+          const uint32_t s = s0 + lane;
+          uint32_t cnt = 0;
+          bool internal = true;
+          if (s < e) {  // leading entries only: an image read during a split may have gaps
+            internal = !(hi32(rc.nodes[s * NW + NW - 1]) & kLeafBit);
+            while (cnt < node_t::kData && lo32(rc.nodes[s * NW + cnt]) != kEmpty) ++cnt;
+          }
+          all_internal = all_internal && __builtin_amdgcn_ballot_w64(!internal) == 0;
+          uint32_t incl = cnt;  // inclusive scan over the wave
+          for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t up = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lane >= d ? lane - d : lane) << 2), (int)incl);
+            if (lane >= d) incl += up;
+          }
+          if (s < e) rc.first_child[s] = (uint16_t)(next + incl - cnt);
+          next += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        }
+        const bool fits = all_internal && next <= rc_t::kMaxNodes && next > e;
+        for (uint32_t s = b + lane; s < e; s += 64) {
+          if (!fits) {
+            rc.first_child[s] = rc_t::kNoChild;
+            continue;
+          }
+          const uint32_t fc = rc.first_child[s];
+          for (uint32_t l = 0; l < node_t::kData; ++l) {
+            const uint64_t pr = rc.nodes[s * NW + l];
+            if (lo32(pr) == kEmpty) break;
+            rc.index[fc + l] = hi32(pr);
+          }
+        }
+        if (lane == 0) {
+          if (fits) {
+            rc.count = next;
+          } else {
+            rc.stop = 1;
+            if (!all_internal) rc.count = b;  // a level with leaves is not routed through
+          }
+        }
+      }
+      __syncthreads();
+      if (rc.stop) break;
+      b = e;
+      e = rc.count;
+    }
+  }
+
+  // Routes `key` through the cached levels, one key per lane (no cross-lane
+  // operation: any subset of lanes may call it). Returns the first node to
+  // read from global memory and, in `parent`, the node it was reached from
+  // (kNone after a right move). A torn or stale image only routes left of the
+  // right node; the root is returned if an image names no node of the pool.
+  MVBT_DEV uint32_t route_lane(const route_cache<NW>& rc, uint32_t key, uint32_t& parent) const {
+    using rc_t = route_cache<NW>;
+    uint32_t s = 0;
+    parent = kNone;
+    for (uint32_t step = 0; step < 2 * rc_t::kMaxNodes; ++step) {
+      const uint64_t* img = rc.nodes + s * NW;
+      const uint64_t m = img[node_t::kMetaLane];
+      uint32_t next;
+      if (key >= node_t::high_key(m)) {
+        next = node_t::sibling(m);
+        parent = kNone;
+        if (s + 1 < rc.count && rc.index[s + 1] == next) {
+          ++s;
+          continue;
+        }
+      } else {
+        uint32_t l = 0;  // last entry whose pivot is <= key (entry 0 routes from the node's low bound)
+#pragma unroll
+        for (uint32_t e = 1; e < node_t::kData; ++e) {
+          const uint32_t k = lo32(img[e]);
+          if (k != kEmpty && key >= k) l = e;
+        }
+        next = hi32(img[l]);
+        parent = rc.index[s];
+        const uint32_t fc = rc.first_child[s];
+        if (fc != rc_t::kNoChild && fc + l < rc.count && rc.index[fc + l] == next) {
+          s = fc + l;
+          continue;
+        }
+      }
+      if (next < cap_) return next;
+      break;
+    }
+    parent = kNone;
+    return root_;
+  }
+
   // ---- find -----------------------------------------------------------------
   template <bool CONC>
-  MVBT_DEV uint32_t find(uint32_t key, uint32_t ts, const tile_t& t) const {
+  MVBT_DEV uint32_t find(uint32_t key, uint32_t ts, const tile_t& t, uint32_t start = kNone) const {
     if (!valid_key(key)) return kEmpty;
     node_t n;
-    uint32_t idx = descend<CONC>(key, n, t);
+    uint32_t idx = descend<CONC>(key, n, t, nullptr, start);
     return find_in<CONC>(key, ts, n, idx, t);
   }
   // Plain trees rewrite a leaf in place (pairs shift on insert and erase), a
@@ -682,11 +826,11 @@ struct tree_view : tree_ptrs {
   // Writes min(count, cap) pairs to out[0..) and returns the true count.
   template <bool CONC>
   MVBT_DEV uint32_t range(uint32_t lo, uint32_t hi, uint32_t ts, uint64_t* out, uint32_t cap,
-                          const tile_t& t) const {
+                          const tile_t& t, uint32_t start = kNone) const {
     if (lo == 0) lo = 1;
     if (hi <= lo) return 0;
     node_t n;
-    uint32_t idx = descend<CONC>(lo, n, t);
+    uint32_t idx = descend<CONC>(lo, n, t, nullptr, start);
     return range_in<CONC>(lo, hi, ts, n, idx, out, cap, t);
   }
   // The range query once `lo` (>= 1, below hi) has been routed to leaf `idx`,
