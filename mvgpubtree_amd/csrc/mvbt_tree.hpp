@@ -296,6 +296,14 @@ struct tree_view : tree_ptrs {
     order_release();
     if (t.leader()) atomic_and32(lock_word(idx), ~kLockBit);
   }
+  // Both locks after one wait for the tile's earlier stores.
+  MVBT_DEV void unlock2(uint32_t a, uint32_t b, const tile_t& t) const {
+    order_release();
+    if (t.leader()) {
+      atomic_and32(lock_word(a), ~kLockBit);
+      atomic_and32(lock_word(b), ~kLockBit);
+    }
+  }
 
   // ---- snapshots ------------------------------------------------------------
   // Takes a snapshot: returns the id s; writes after this call are stamped > s.
@@ -960,14 +968,14 @@ struct tree_view : tree_ptrs {
     uint32_t pivot = n.split_pivot(t);
     node_t yn = n.upper_half(m, pack(0u, kNone), t);
     yn.store(at(y), t);
-    order_release();
+    order_release();  // y is complete before anything links to it
+    // The left half and the parent's new entry go out in one round: a reader
+    // that sees either one reaches the complete y.
     n.keep_lower_half(pivot, y, t);
     n.store(at(idx), t);  // metadata lane keeps the lock bit
-    order_release();
     pn.insert(pivot, y, t);
     pn.store_body(at(parent), t);
-    unlock(parent, t);
-    unlock(idx, t);
+    unlock2(parent, idx, t);
     return kAtParent;
   }
 
@@ -998,6 +1006,11 @@ struct tree_view : tree_ptrs {
   // half that covers it before either half is stored, and kInserted is
   // returned: the insert that needed the split costs no second descent, lock
   // and store round.
+  // Store rounds (each ends with one wait): {epoch copy, right half y};
+  // {version word} when a copy was made; {left half, parent's new entry};
+  // then both unlocks. The orders that readers rely on hold: the copy before
+  // the version word naming it, the version word before the data it guards,
+  // y before any link to it.
   MVBT_DEV uint32_t split_leaf(node_t& n, uint32_t idx, uint32_t parent, write_ctx& c, const tile_t& t,
                                uint32_t key, uint32_t value) const {
     if (parent == kNone) {
@@ -1017,10 +1030,23 @@ struct tree_view : tree_ptrs {
       return kAtRoot;
     }
     uint32_t y = alloc(c, t);
-    if (y == kNone || !open_epoch(n, idx, c, t)) {
-      unlock(parent, t);
-      unlock(idx, t);
-      return kOutOfMemory;  // an allocated y is simply never linked; counted as leaked
+    bool copied = false;
+    if constexpr (VER) {
+      if (y != kNone && node_t::stamp(n.version(t)) != c.cur) {  // new epoch (as open_epoch)
+        const uint32_t cp = alloc(c, t, true);
+        if (cp == kNone) {
+          y = kNone;  // an allocated y is simply never linked; counted as leaked
+        } else {
+          n.store_unlocked_copy(at(cp), t);
+          retire(c, cp, t);
+          n.set_lane(node_t::kVerLane, pack(c.cur, cp), t);
+          copied = true;
+        }
+      }
+    }
+    if (y == kNone) {
+      unlock2(parent, idx, t);
+      return kOutOfMemory;
     }
     uint64_t m = n.meta(t);
     uint32_t pivot = n.split_pivot(t);
@@ -1032,15 +1058,17 @@ struct tree_view : tree_ptrs {
     yn.store(at(y), t);
     order_release();
     if constexpr (VER) {
+      if (copied) {
+        if (t.leader()) st_coherent(at(idx) + node_t::kVerLane, ver);
+        order_release();
+      }
       if (t.rank() != node_t::kVerLane) st_coherent(at(idx) + t.rank(), n.p);
     } else {
       n.store(at(idx), t);
     }
-    order_release();
     pn.insert(pivot, y, t);
     pn.store_body(at(parent), t);
-    unlock(parent, t);
-    unlock(idx, t);
+    unlock2(parent, idx, t);
     return kInserted;
   }
 
