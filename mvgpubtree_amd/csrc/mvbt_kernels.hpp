@@ -130,7 +130,10 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
     tv.template descend_chunk<true>(key, q, leaf, par, t);
     // Leaves that take keys of this chunk are written together; the keys it
     // hands back (lock busy, split needed, leaf moved) go one at a time.
+    // ctx.want is the chunk's exact count: refills for version copies take
+    // the whole chunk's need at once.
     uint32_t res = 0;
+    ctx.want = (uint32_t)__builtin_popcountll(q);
     q = tv.template write_chunk<OP>(key, val, q, leaf, res, ctx, t);
     while (q) {
       uint32_t j = (uint32_t)lowest_bit(q);
@@ -185,6 +188,7 @@ __global__ void __launch_bounds__(256)
       const uint32_t val = (IR && is_w) ? wvals[i] : 0u;
       uint32_t leaf = kNone, par = kNone, res = 0;
       tv.template descend_chunk<true>(key, q, leaf, par, t);
+      ctx.want = (uint32_t)__builtin_popcountll(q);
       q = tv.template write_chunk<OP>(key, val, q, leaf, res, ctx, t);
       while (q) {
         const uint32_t j = (uint32_t)lowest_bit(q);
