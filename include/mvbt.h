@@ -60,7 +60,7 @@ typedef enum mvbt_kind {
 typedef struct mvbt_config {
   uint32_t kind;            /* mvbt_kind */
   uint32_t node_width;      /* pairs per node: 16 (128 B), 32 or 64; 0 = 16 */
-  uint64_t pool_nodes;      /* node slots (< 2^30); 0 = 1<<22 */
+  uint64_t pool_nodes;      /* node slots (< 2^29); 0 = 1<<22 */
   uint64_t retire_capacity; /* retire ring entries (power of two); 0 = 1<<22 */
   int32_t device;           /* HIP device ordinal; -1 = current device */
   uint32_t reserved[3];

@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
     if (op.type == kOpRange && rkey == 0) rkey = 1;
     const bool route = mine && valid_key(rkey) && !(op.type == kOpRange && op.arg <= rkey);
     uint32_t leaf = kNone, par = kNone, res = 0;
-    tv.template descend_chunk<true>(rkey, t.ballot(route), leaf, par, t);
+    tv.template descend_chunk<true>(rkey, t.ballot(route), leaf, par, t, t.ballot(is_w));
     // writes: batched leaf phase, then the keys it hands back one by one
     ctx.cur = clock0 + b + 1;
     const uint64_t qw = t.ballot(is_w);

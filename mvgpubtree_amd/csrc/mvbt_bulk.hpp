@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256) bulk_build_kernel(tree_view<NW, VER> tv, 
         const unsigned long long first = (unsigned long long)r * plan.span[top];
         p = pack(r == 0 ? 0u : keys[first], (uint32_t)(plan.base[top] + r));
       } else if (r == node_t::kMetaLane) {
-        p = pack(kEmpty, kNoSibling);
+        p = pack(kEmpty, kNoSibling | (plan.levels == 1 ? kAboveLeaves : 0u));
       } else if (VER && r == node_t::kVerLane) {
         p = pack(0u, kNone);
       }
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(256) bulk_build_kernel(tree_view<NW, VER> tv, 
           const unsigned long long c = c0 + r;
           p = pack(c == 0 ? 0u : keys[c * span], (uint32_t)(plan.base[lv - 1] + c));
         } else if (r == node_t::kMetaLane) {
-          p = pack(last ? kEmpty : keys[(c0 + fill) * span], sib);
+          p = pack(last ? kEmpty : keys[(c0 + fill) * span], sib | (lv == 1 ? kAboveLeaves : 0u));
         } else if (VER && r == node_t::kVerLane) {
           p = pack(0u, kNone);
         }

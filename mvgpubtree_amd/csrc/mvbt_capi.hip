@@ -330,12 +330,13 @@ mvbt_status walk_tree(const host_walk& w, std::vector<mvbt_pair>* pairs, uint64_
   std::vector<uint32_t> level{0u};  // root
   uint32_t h = 0;
   uint64_t keys = 0;
+  bool parent_above = false;  // the level above flagged its children as leaves
   while (true) {
     ++h;
     if (h > 64) return bad("tree deeper than 64 levels");
     // Follow the sibling chain from the left-most node and compare with `level`.
     std::vector<uint32_t> next;
-    bool leaf_level = false;
+    bool leaf_level = false, above_level = false;
     uint32_t prev_high = 0;
     for (size_t li = 0; li < level.size(); ++li) {
       uint32_t idx = level[li];
@@ -344,8 +345,17 @@ mvbt_status walk_tree(const host_walk& w, std::vector<mvbt_pair>* pairs, uint64_
       uint32_t flags = (uint32_t)(m >> 32);
       if (flags & kLockBit) return bad("node " + std::to_string(idx) + " is locked");
       bool leaf = (flags & kLeafBit) != 0;
-      if (li == 0) leaf_level = leaf;
-      else if (leaf != leaf_level) return bad("mixed leaf/internal level");
+      bool above = (flags & kAboveLeaves) != 0;
+      if (li == 0) {
+        leaf_level = leaf;
+        above_level = above;
+        if (h > 1 && leaf != parent_above) return bad("level-1 flag does not match the level below");
+      } else if (leaf != leaf_level) {
+        return bad("mixed leaf/internal level");
+      } else if (above != above_level) {
+        return bad("mixed level-1 flags in one level");
+      }
+      if (leaf && above) return bad("leaf " + std::to_string(idx) + " flagged as level 1");
       uint32_t high = (uint32_t)m;
       uint32_t sib = flags & kPtrMask;
       if (li + 1 < level.size()) {
@@ -379,6 +389,7 @@ mvbt_status walk_tree(const host_walk& w, std::vector<mvbt_pair>* pairs, uint64_
       prev_high = high;
     }
     if (leaf_level) break;
+    parent_above = above_level;
     level.swap(next);
   }
   if (num_keys) *num_keys = keys;
@@ -485,7 +496,7 @@ mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
   if (!table_for(cfg.node_width, true)) return fail(MVBT_ERR_INVALID_ARG, "node_width must be 16, 32 or 64");
   if (cfg.pool_nodes == 0) cfg.pool_nodes = 1ull << 22;
   if (cfg.pool_nodes < 4 || cfg.pool_nodes >= kMaxNodes)
-    return fail(MVBT_ERR_INVALID_ARG, "pool_nodes must be in [4, 2^30-1)");
+    return fail(MVBT_ERR_INVALID_ARG, "pool_nodes must be in [4, 2^29-1)");
   if (cfg.retire_capacity == 0) cfg.retire_capacity = 1ull << 22;
   if (!is_pow2(cfg.retire_capacity)) return fail(MVBT_ERR_INVALID_ARG, "retire_capacity must be a power of two");
 

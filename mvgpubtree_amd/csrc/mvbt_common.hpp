@@ -36,9 +36,10 @@ constexpr uint32_t kLatest = 0xFFFFFFFFu;    // snapshot id meaning "latest vers
 constexpr uint32_t kNone = 0xFFFFFFFFu;      // no older version
 constexpr uint32_t kLockBit = 1u << 31;      // metadata value word: lock
 constexpr uint32_t kLeafBit = 1u << 30;      // metadata value word: leaf
-constexpr uint32_t kPtrMask = (1u << 30) - 1;
+constexpr uint32_t kAboveLeaves = 1u << 29;  // metadata value word: internal node whose children are leaves
+constexpr uint32_t kPtrMask = (1u << 29) - 1;
 constexpr uint32_t kNoSibling = kPtrMask;    // right-most node of a level
-constexpr uint32_t kMaxNodes = kPtrMask;     // node indices must fit 30 bits
+constexpr uint32_t kMaxNodes = kPtrMask;     // node indices must fit 29 bits
 
 MVBT_HD bool valid_key(uint32_t k) { return k != 0u && k != kEmpty; }
 

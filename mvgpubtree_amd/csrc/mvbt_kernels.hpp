@@ -31,7 +31,7 @@ __global__ void init_kernel(tree_view<NW, VER> tv) {
   uint32_t r = t.rank();
   uint64_t rp, lp;
   if (r == node_t::kMetaLane) {
-    rp = pack(kEmpty, kNoSibling);
+    rp = pack(kEmpty, kNoSibling | kAboveLeaves);
     lp = pack(kEmpty, kNoSibling | kLeafBit);
   } else if (VER && r == node_t::kVerLane) {
     rp = pack(0u, kNone);
@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
     // Route the whole chunk first (one latency per level for all its keys),
     // then write key by key starting from the routed leaf.
     uint32_t leaf = kNone, par = kNone;
-    tv.template descend_chunk<true>(key, q, leaf, par, t);
+    tv.template descend_chunk<true>(key, q, leaf, par, t, q);
     // Leaves that take keys of this chunk are written together; the keys it
     // hands back (lock busy, split needed, leaf moved) go one at a time.
     // ctx.want is the chunk's exact count: refills for version copies take
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256)
       const uint32_t key = is_w ? wkeys[i] : 0u;
       const uint32_t val = (IR && is_w) ? wvals[i] : 0u;
       uint32_t leaf = kNone, par = kNone, res = 0;
-      tv.template descend_chunk<true>(key, q, leaf, par, t);
+      tv.template descend_chunk<true>(key, q, leaf, par, t, q);
       ctx.want = (uint32_t)__builtin_popcountll(q);
       q = tv.template write_chunk<OP>(key, val, q, leaf, res, ctx, t);
       while (q) {

@@ -7,7 +7,9 @@
 //   lanes [0, kData)   sorted {key, value} in leaves, {pivot, child} in internal
 //                      nodes; empty lanes hold {kEmpty, kEmpty} at the end
 //   lane NW-2          (versioned only) version word {stamp, older-version index}
-//   lane NW-1          metadata {high key, lock | leaf | 30-bit right sibling}
+//   lane NW-1          metadata {high key, lock | leaf | level-1 | 29-bit right
+//                      sibling}; level-1 marks an internal node whose children
+//                      are leaves
 //
 // Search and update are ballot/shuffle based, as in the reference
 // (find_next_lane versioned_node.hpp:214-236, get_key_value_from_node :243-271,
@@ -58,6 +60,9 @@ struct node {
   MVBT_DEV static uint32_t high_key(uint64_t meta) { return lo32(meta); }
   MVBT_DEV static uint32_t sibling(uint64_t meta) { return hi32(meta) & kPtrMask; }
   MVBT_DEV static bool is_leaf(uint64_t meta) { return (hi32(meta) & kLeafBit) != 0; }
+  // Internal node whose children are leaves (level 1). A node's level never
+  // changes except the root's, which clears the flag when it first splits.
+  MVBT_DEV static bool above_leaves(uint64_t meta) { return (hi32(meta) & kAboveLeaves) != 0; }
   MVBT_DEV static bool is_locked(uint64_t meta) { return (hi32(meta) & kLockBit) != 0; }
   MVBT_DEV static uint32_t stamp(uint64_t ver) { return lo32(ver); }
   MVBT_DEV static uint32_t older(uint64_t ver) { return hi32(ver); }
@@ -162,7 +167,7 @@ struct node {
     if (data_lane(r)) {
       y.p = (r + kHalf < kData) ? moved : kEmptyPair;
     } else if (r == kMetaLane) {
-      y.p = pack(high_key(my_meta), hi32(my_meta) & (kPtrMask | kLeafBit));
+      y.p = pack(high_key(my_meta), hi32(my_meta) & (kPtrMask | kLeafBit | kAboveLeaves));
     } else {
       y.p = right_version;  // version lane (VER only)
     }
@@ -172,7 +177,7 @@ struct node {
   MVBT_DEV void keep_lower_half(uint32_t pivot, uint32_t right, const tile_t& t) {
     uint32_t r = t.rank();
     if (data_lane(r) && r >= kHalf) p = kEmptyPair;
-    if (r == kMetaLane) p = pack(pivot, (hi32(p) & (kLockBit | kLeafBit)) | right);
+    if (r == kMetaLane) p = pack(pivot, (hi32(p) & (kLockBit | kLeafBit | kAboveLeaves)) | right);
   }
 };
 

@@ -418,10 +418,16 @@ struct tree_view : tree_ptrs {
   // its parent (kNone when the leaf was reached by a right move), or kNone as
   // leaf when the walk left the pool (kErrCorrupt is flagged). Routing only:
   // writers re-check both under lock, readers validate leaf versions.
+  // Lanes in `early` (writes) stop at the level above the leaves and take the
+  // leaf from their parent's entry without reading it: a writer locks the
+  // leaf before it reads it anyway, and checks under the lock that it is a
+  // leaf covering the key (a torn read of a root being split can name an
+  // internal node here). Readers (not in `early`) always end on a node read
+  // as a leaf.
   static constexpr int kGroup = NW < 8 ? NW : 8;
   template <bool CONC>
   MVBT_DEV void descend_chunk(uint32_t key, uint64_t todo, uint32_t& leaf, uint32_t& parent,
-                              const tile_t& t) const {
+                              const tile_t& t, uint64_t early = 0) const {
     const uint32_t r = t.rank();
     uint32_t cur = root_, par = kNone;
     bool corrupt = false;
@@ -450,6 +456,7 @@ struct tree_view : tree_ptrs {
           const uint64_t m = n.meta(t);
           uint32_t next = kNone, npar = kNone;
           bool done = false;
+          bool arrived = false;  // next is the leaf: writers lock it before reading it
           if (k >= node_t::high_key(m)) {
             next = node_t::sibling(m);
           } else if (node_t::is_leaf(m)) {
@@ -457,6 +464,7 @@ struct tree_view : tree_ptrs {
           } else {
             next = n.child_for(k, t);
             npar = idx;
+            arrived = node_t::above_leaves(m) && ((early >> (base + j)) & 1);
           }
           if (!done && next >= cap_) {
             corrupt = true;
@@ -467,7 +475,7 @@ struct tree_view : tree_ptrs {
             cur = next;
             par = npar;
           }
-          if (done) active &= ~(1ull << j);
+          if (done || arrived) active &= ~(1ull << j);
         }
       }
     }
@@ -926,7 +934,14 @@ struct tree_view : tree_ptrs {
     if (node_t::data_lane(rk)) {
       n.p = (rk == 0) ? pack(k0, left) : (rk == 1) ? pack(pivot, right) : node_t::kEmptyPair;
     }
-    n.store_body(at(r), t);
+    if (node_t::above_leaves(m)) {
+      // the root is now above internal nodes: its metadata loses the flag
+      // (stored whole while the lock is held; the lock bit stays set)
+      if (rk == node_t::kMetaLane) n.p = pack(lo32(m), hi32(m) & ~kAboveLeaves);
+      if (!VER || rk != node_t::kVerLane) st_coherent(at(r) + rk, n.p);
+    } else {
+      n.store_body(at(r), t);
+    }
     unlock(r, t);
     return kAtRoot;
   }
@@ -1281,8 +1296,8 @@ struct tree_view : tree_ptrs {
           }
         }
         // 4. version copies (new epoch), then their version words
+        uint32_t cp = kNone;  // leader lane: copy slot of its leaf
         if constexpr (VER) {
-          uint32_t cp = kNone;  // leader lane: copy slot of its leaf
 #pragma unroll
           for (int i = 0; i < kLeafBatch; ++i) {
             if (!((need_copy >> i) & 1)) continue;
@@ -1317,10 +1332,17 @@ struct tree_view : tree_ptrs {
             if (OP == 0) n.insert(t.shfl(key, j), t.shfl(value, j), t);
             else n.erase(t.shfl(key, j), t);
           }
-          // (the shuffle must run on every lane of the tile: a permute reads
+          // The whole node goes out as one full 128-byte line: the metadata
+          // lane as loaded under the lock (unchanged while it is held) and
+          // the version word as just published.
+          // (the shuffles must run on every lane of the tile: a permute reads
           // nothing from a disabled source lane, and the leader may be one)
           const uint32_t target = t.shfl(leaf, gl[i]);
-          if (node_t::data_lane(r)) st_coherent(at(target) + r, n.p);
+          if constexpr (VER) {
+            const uint32_t cpi = t.shfl(cp, gl[i]);
+            if (((need_copy >> i) & 1) && r == node_t::kVerLane) n.p = pack(c.cur, cpi);
+          }
+          st_coherent(at(target) + r, n.p);
         }
         // 6. unlock every locked leaf
         order_release();

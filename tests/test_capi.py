@@ -68,3 +68,6 @@ def test_invalid_config_is_rejected_without_device_work():
     assert lib.mvbt_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
     cfg = mv._Config(kind=1, node_width=16, pool_nodes=1 << 30, retire_capacity=1024, device=0)
     assert lib.mvbt_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
+    # node indices are 29-bit (the metadata word's top bits are flags)
+    cfg = mv._Config(kind=1, node_width=16, pool_nodes=(1 << 29) - 1, retire_capacity=1024, device=0)
+    assert lib.mvbt_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
