@@ -182,7 +182,7 @@ class gpu_versioned_btree : public detail::tree_base<B, MVBT_VERSIONED> {
                            concurrent ? MVBT_CONCURRENT : 0),
           "range_query");
   }
-  // Returns the snapshot id; blocks until taken (reference take_snapshot).
+  // Returns the snapshot id (reference take_snapshot); stream-ordered, see mvbt_take_snapshot.
   size_type take_snapshot(hipStream_t stream = 0) {
     uint32_t id = 0;
     check(mvbt_take_snapshot(this->h_, stream, &id), "take_snapshot");

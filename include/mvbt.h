@@ -51,6 +51,8 @@ typedef enum mvbt_kind {
 #define MVBT_FLAG_VERSION_GAP 0x8u     /* read at a snapshot older than the data */
 #define MVBT_FLAG_RANGE_OVERFLOW 0x10u /* a range query matched more than L pairs */
 #define MVBT_FLAG_CORRUPT 0x20u        /* a traversal left the pool or hit its step bound */
+#define MVBT_FLAG_SNAPSHOT_ID 0x40u    /* a snapshot id returned without waiting differs from the
+                                          device clock (the library's own consistency check) */
 
 /* Operation flags. */
 #define MVBT_CONCURRENT 0x1u /* readers may run while writers run on another stream
@@ -128,9 +130,14 @@ mvbt_status mvbt_range_query(mvbt_tree* tree, const uint32_t* lower, const uint3
                              uint32_t snapshot, void* stream, uint32_t flags);
 
 /* Replaces take_snapshot(stream) (gpu_versioned_blink_tree.hpp:297-302): returns
- * the id s; writes issued after it are not visible at s. Blocks until the
- * snapshot kernel has run, like the reference. The snapshot stays live (its
- * versions are kept) until mvbt_release_snapshot. Versioned trees only. */
+ * the id s; writes issued after it (on any stream, through this API) are not
+ * visible at s. The reference blocks until its snapshot kernel has run; here
+ * the call returns at once with the id from the host's copy of the clock and
+ * the snapshot kernel is stream-ordered: work on `stream` and writes on other
+ * streams are ordered after it. Once a device view has been handed out (user
+ * kernels may take snapshots), the call blocks and reads the id back. The
+ * snapshot stays live (its versions are kept) until mvbt_release_snapshot.
+ * Versioned trees only. */
 mvbt_status mvbt_take_snapshot(mvbt_tree* tree, void* stream, uint32_t* snapshot_id);
 
 /* No reference counterpart (the reference never frees in-place version copies):

@@ -21,7 +21,7 @@ from dataclasses import dataclass
 __all__ = [
     "GpuBlinkTree", "GpuVersionedBTree", "MvbtError", "LATEST", "NOT_FOUND",
     "FLAG_POOL_EXHAUSTED", "FLAG_RETIRE_OVERFLOW", "FLAG_INVALID_KEY", "FLAG_VERSION_GAP",
-    "FLAG_RANGE_OVERFLOW", "FLAG_CORRUPT",
+    "FLAG_RANGE_OVERFLOW", "FLAG_CORRUPT", "FLAG_SNAPSHOT_ID",
     "CONCURRENT", "library_path", "load_library", "EXPORTED_SYMBOLS",
 ]
 
@@ -39,6 +39,7 @@ FLAG_INVALID_KEY = 0x4
 FLAG_VERSION_GAP = 0x8
 FLAG_RANGE_OVERFLOW = 0x10
 FLAG_CORRUPT = 0x20
+FLAG_SNAPSHOT_ID = 0x40
 PLAIN, VERSIONED = 0, 1
 
 _STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "UNSUPPORTED", 5: "DEVICE_FLAGS"}

@@ -50,10 +50,33 @@ struct mvbt_tree {
   uint32_t* free_stack = nullptr;
   uint32_t* spill = nullptr;
   uint32_t* retire_ring = nullptr;
-  snapshot_record* snap_rec = nullptr;  // pinned host slot
+  snapshot_record* snap_rec = nullptr;  // pinned host slot (synchronous snapshots)
   std::mutex mu;
-  std::map<uint32_t, uint32_t> live;                  // live snapshot id -> refcount
-  std::vector<std::pair<uint32_t, uint64_t>> bounds;  // (snapshot id, ring pos at that epoch edge)
+  std::map<uint32_t, uint32_t> live;  // live snapshot id -> refcount
+  // Epoch edges of live snapshots: the retire-ring position when each was
+  // taken, known on the host (slot < 0) or held in device slot snap_pos[slot]
+  // (stream-ordered snapshots).
+  struct bound {
+    uint32_t id;
+    uint64_t pos;
+    int32_t slot;
+  };
+  std::vector<bound> bounds;
+  // Stream-ordered snapshots. Every kernel that advances the device clock
+  // (snapshot, mixed-call and batched setup) is ordered after the previous
+  // one through clock_ev, so the clock advances in host call order and
+  // `clock` (the host's copy of it) names each snapshot without a round trip.
+  // Writes on another stream wait for clock_ev too: a write issued after
+  // take_snapshot returns is stamped after the snapshot. The copy is exact
+  // until a device view is handed out (user kernels may then take
+  // snapshots); from then on take_snapshot reads the id back synchronously.
+  static constexpr uint32_t kSnapSlots = 4096;
+  unsigned long long* snap_pos = nullptr;  // device: kSnapSlots ring positions
+  uint64_t snap_next = 0;                  // next slot (mod kSnapSlots)
+  uint32_t clock = 0;
+  hipEvent_t clock_ev = nullptr;
+  hipStream_t clock_ev_stream = nullptr;
+  bool clock_ev_set = false;
   // Free-list maintenance (spill merge, reclamation) must not overlap a
   // writer on another stream: writers wait for the last maintenance event,
   // and maintenance runs only when every other stream's last write is done.
@@ -235,8 +258,9 @@ bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 
 // Launches the two reclamation kernels: frees ring entries [head, end_pos)
 // (end_pos = 0 frees none) and returns parked spill slots to the free stack.
-mvbt_status launch_maintenance(mvbt_tree* t, hipStream_t s, unsigned long long end_pos) {
-  hipLaunchKernelGGL(reclaim_prep_kernel, dim3(1), dim3(64), 0, s, t->st, end_pos);
+mvbt_status launch_maintenance(mvbt_tree* t, hipStream_t s, unsigned long long end_pos,
+                               const unsigned long long* end_from = nullptr) {
+  hipLaunchKernelGGL(reclaim_prep_kernel, dim3(1), dim3(64), 0, s, t->st, end_pos, end_from);
   MVBT_HIP(hipGetLastError());
   hipLaunchKernelGGL(reclaim_copy_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->st,
                      t->retire_ring, t->spill, t->free_stack);
@@ -254,8 +278,25 @@ bool other_writers_busy(mvbt_tree* t, hipStream_t s) {
 }
 
 // Before a write kernel on stream s (caller holds t->mu).
+// Orders work on stream s after the last clock-advancing kernel when that ran
+// on another stream.
+mvbt_status after_clock(mvbt_tree* t, hipStream_t s) {
+  if (t->clock_ev_set && t->clock_ev_stream != s) MVBT_HIP(hipStreamWaitEvent(s, t->clock_ev, 0));
+  return MVBT_OK;
+}
+// Records that a clock-advancing kernel was just launched on stream s.
+mvbt_status clock_advanced(mvbt_tree* t, hipStream_t s, uint32_t by) {
+  if (!t->clock_ev) MVBT_HIP(hipEventCreateWithFlags(&t->clock_ev, hipEventDisableTiming));
+  MVBT_HIP(hipEventRecord(t->clock_ev, s));
+  t->clock_ev_stream = s;
+  t->clock_ev_set = true;
+  t->clock += by;
+  return MVBT_OK;
+}
+
 mvbt_status before_write(mvbt_tree* t, hipStream_t s) {
   t->fresh = false;
+  if (mvbt_status rc = after_clock(t, s)) return rc;
   if (t->maint_ev) MVBT_HIP(hipStreamWaitEvent(s, t->maint_ev, 0));
   if ((t->spill_maybe || t->external_writers) && !other_writers_busy(t, s))
     return launch_maintenance(t, s, 0);
@@ -521,6 +562,7 @@ mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
     if (t->free_stack) (void)hipFree(t->free_stack);
     if (t->spill) (void)hipFree(t->spill);
     if (t->retire_ring) (void)hipFree(t->retire_ring);
+    if (t->snap_pos) (void)hipFree(t->snap_pos);
     if (t->snap_rec) (void)hipHostFree(t->snap_rec);
   };
   const size_t node_bytes = (size_t)t->nw * sizeof(uint64_t);
@@ -529,6 +571,7 @@ mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
       hipMalloc(&t->free_stack, cfg.pool_nodes * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&t->spill, cfg.pool_nodes * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&t->retire_ring, cfg.retire_capacity * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&t->snap_pos, mvbt_tree::kSnapSlots * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc(&t->snap_rec, sizeof(snapshot_record), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     cleanup();
@@ -565,9 +608,11 @@ mvbt_status mvbt_destroy(mvbt_tree* t) {
   (void)hipFree(t->free_stack);
   (void)hipFree(t->spill);
   (void)hipFree(t->retire_ring);
+  (void)hipFree(t->snap_pos);
   (void)hipHostFree(t->snap_rec);
   for (auto& kv : t->write_ev) (void)hipEventDestroy(kv.second);
   if (t->maint_ev) (void)hipEventDestroy(t->maint_ev);
+  if (t->clock_ev) (void)hipEventDestroy(t->clock_ev);
   destroy_graph(t);
   if (t->job) (void)hipFree(t->job);
   if (t->b_nwrites) (void)hipFree(t->b_nwrites);
@@ -639,13 +684,34 @@ mvbt_status mvbt_take_snapshot(mvbt_tree* t, void* stream, uint32_t* snapshot_id
   std::lock_guard<std::mutex> lk(t->mu);
   hipStream_t s = (hipStream_t)stream;
   t->fresh = false;
-  hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(64), 0, s, t->st, t->snap_rec);
+  if (mvbt_status rc = after_clock(t, s)) return rc;
+  // Stream-ordered: the id is the host's copy of the clock; the kernel
+  // records the epoch edge in a device slot (one not held by a live bound).
+  const int32_t slot = (int32_t)(t->snap_next % mvbt_tree::kSnapSlots);
+  bool slot_free = true;
+  for (const auto& b : t->bounds) slot_free = slot_free && b.slot != slot;
+  if (!t->external_writers && slot_free) {
+    const uint32_t id = t->clock;
+    if (id == MVBT_LATEST) return fail(MVBT_ERR_UNSUPPORTED, "snapshot ids exhausted");
+    hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(64), 0, s, t->st, nullptr, t->snap_pos + slot, id);
+    MVBT_HIP(hipGetLastError());
+    if (mvbt_status rc = clock_advanced(t, s, 1)) return rc;
+    t->snap_next += 1;
+    t->live[id] += 1;
+    t->bounds.push_back({id, 0, slot});
+    *snapshot_id = id;
+    return MVBT_OK;
+  }
+  // Synchronous: the id is read back from the device.
+  hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(64), 0, s, t->st, t->snap_rec, nullptr, kNone);
   MVBT_HIP(hipGetLastError());
   MVBT_HIP(hipStreamSynchronize(s));
   uint32_t id = t->snap_rec->id;
   if (id == MVBT_LATEST) return fail(MVBT_ERR_UNSUPPORTED, "snapshot ids exhausted");
+  if (mvbt_status rc = clock_advanced(t, s, 0)) return rc;
+  t->clock = id + 1;
   t->live[id] += 1;
-  t->bounds.emplace_back(id, t->snap_rec->ring_pos);
+  t->bounds.push_back({id, t->snap_rec->ring_pos, -1});
   *snapshot_id = id;
   return MVBT_OK;
 }
@@ -667,23 +733,29 @@ mvbt_status mvbt_reclaim(mvbt_tree* t, void* stream) {
   // Copies retired before the edge of the oldest live snapshot are garbage;
   // with no live snapshot, every retired copy is.
   unsigned long long end = ~0ull;
+  const unsigned long long* end_from = nullptr;  // the edge, when it is held on the device
   if (!t->live.empty()) {
     uint32_t oldest = t->live.begin()->first;
     end = 0;
     for (const auto& b : t->bounds)
-      if (b.first == oldest) end = b.second;
+      if (b.id == oldest) {
+        if (b.slot >= 0) end_from = t->snap_pos + b.slot;
+        else end = b.pos;
+      }
   }
   // Forget edges no longer needed.
-  std::vector<std::pair<uint32_t, uint64_t>> keep;
+  std::vector<mvbt_tree::bound> keep;
   for (const auto& b : t->bounds)
-    if (t->live.count(b.first)) keep.push_back(b);
+    if (t->live.count(b.id)) keep.push_back(b);
   t->bounds.swap(keep);
   hipStream_t s = (hipStream_t)stream;
   // Writers on other streams are ordered after this by maint_ev; writers
   // already running on them must have finished (documented precondition).
   for (auto& kv : t->write_ev)
     if (kv.first != s) MVBT_HIP(hipStreamWaitEvent(s, kv.second, 0));
-  return launch_maintenance(t, s, end);
+  // a device-held edge is written by a snapshot kernel, possibly on another stream
+  if (mvbt_status rc = after_clock(t, s)) return rc;
+  return launch_maintenance(t, s, end, end_from);
 }
 
 mvbt_status mvbt_memory_stats(mvbt_tree* t, mvbt_memstats* out) {
@@ -864,6 +936,7 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
     hipLaunchKernelGGL(batch_finish_kernel, dim3(1), dim3(64), 0, s, t->job);
   }
   MVBT_HIP(hipGetLastError());
+  if (mvbt_status rc = clock_advanced(t, s, num_batches)) return rc;
   return after_write(t, s);
 }
 
@@ -941,6 +1014,8 @@ mvbt_status mvbt_concurrent_insert_range(mvbt_tree* t, const uint32_t* keys, con
                            ->insert_range(t, keys, values, num_insertions, lower, upper, num_ranges, nullptr,
                                           reinterpret_cast<uint64_t*>(result), counts, L, s))
     return rc;
+  if (t->versioned)  // the call reserved one snapshot id
+    if (mvbt_status rc = clock_advanced(t, s, 1)) return rc;
   return after_write(t, s);
 }
 
@@ -959,6 +1034,8 @@ mvbt_status mvbt_concurrent_find_erase(mvbt_tree* t, const uint32_t* find_keys, 
                            ->find_erase(t, erase_keys, nullptr, num_erasures, find_keys, nullptr, num_finds,
                                         find_results, nullptr, nullptr, 0, s))
     return rc;
+  if (t->versioned)  // the call reserved one snapshot id
+    if (mvbt_status rc = clock_advanced(t, s, 1)) return rc;
   return after_write(t, s);
 }
 

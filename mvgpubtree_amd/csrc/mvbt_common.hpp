@@ -51,6 +51,7 @@ enum : uint32_t {
   kErrVersionGap = 1u << 3,      // version chain ended above the snapshot
   kErrRangeOverflow = 1u << 4,   // a range query produced more than L pairs
   kErrCorrupt = 1u << 5,         // a pointer left the pool or a loop exceeded its bound
+  kErrSnapshotId = 1u << 6,      // a snapshot id returned without waiting differs from the clock
 };
 
 // Loop bounds: every device loop terminates even on a corrupted structure.

@@ -230,22 +230,34 @@ __global__ void __launch_bounds__(256)
 }
 
 // take_snapshot_kernel (btree_kernels.hpp:473-478): one lane reads the retire
-// ring position, then advances the clock; both go to `rec`.
-__global__ void snapshot_kernel(tree_state* st, snapshot_record* rec) {
+// ring position, then advances the clock. Synchronous form: both go to the
+// host record `rec`. Stream-ordered form (rec null): the ring position goes
+// to the device slot `pos_slot`, read later by reclaim_prep_kernel, and the
+// id the host already returned (`expected`) is checked against the clock.
+__global__ void snapshot_kernel(tree_state* st, snapshot_record* rec, unsigned long long* pos_slot,
+                                uint32_t expected) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   fence_release();
   unsigned long long pos =
       __hip_atomic_load(&st->retire_tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
   uint32_t s = __hip_atomic_fetch_add(&st->clock, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(&rec->ring_pos, pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(&rec->id, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (rec) {
+    __hip_atomic_store(&rec->ring_pos, pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&rec->id, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  } else {
+    __hip_atomic_store(pos_slot, pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (s != expected) atomic_or32(&st->error_flags, kErrSnapshotId);
+  }
 }
 
 // Reclamation, phase 1 (one lane): frees retire-ring entries [head, end) by
 // moving them onto the free stack. Records the live count before freeing as a
 // peak candidate.
-__global__ void reclaim_prep_kernel(tree_state* st, unsigned long long end_pos) {
+__global__ void reclaim_prep_kernel(tree_state* st, unsigned long long end_pos,
+                                    const unsigned long long* end_from) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  // the edge recorded on the device by a stream-ordered snapshot
+  if (end_from) end_pos = __hip_atomic_load(end_from, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   unsigned long long head = st->retire_head, tail = st->retire_tail;
   unsigned long long end = end_pos < tail ? end_pos : tail;
   if (end < head) end = head;
