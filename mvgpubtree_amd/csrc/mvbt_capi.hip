@@ -130,10 +130,10 @@ struct device_guard {
   }
 };
 
-uint32_t grid_for(const mvbt_tree* t, uint64_t n, uint32_t nw, int waves_per_cu) {
-  const uint64_t ops_per_block = kBlock;  // one op per lane per chunk
+uint32_t grid_for(const mvbt_tree* t, uint64_t n, uint32_t nw, int waves_per_cu, int block = kBlock) {
+  const uint64_t ops_per_block = (uint64_t)block;  // one op per lane per chunk
   uint64_t need = (n + ops_per_block - 1) / ops_per_block;
-  uint64_t cap = (uint64_t)t->num_cus * (uint64_t)waves_per_cu * 64 / kBlock;
+  uint64_t cap = (uint64_t)t->num_cus * (uint64_t)waves_per_cu * 64 / (uint64_t)block;
   (void)nw;
   if (need < 1) need = 1;
   return (uint32_t)std::min<uint64_t>(need, std::max<uint64_t>(cap, 1));
@@ -148,12 +148,12 @@ struct impl {
   }
   static mvbt_status find(mvbt_tree* t, const uint32_t* keys, uint32_t* vals, uint64_t n,
                           uint32_t ts, hipStream_t s, bool conc) {
-    uint32_t g = grid_for(t, n, NW, 32);
+    uint32_t g = grid_for(t, n, NW, 32, kFindBlock);
     if (conc)
-      hipLaunchKernelGGL((find_kernel<NW, VER, true>), dim3(g), dim3(kBlock), 0, s,
+      hipLaunchKernelGGL((find_kernel<NW, VER, true>), dim3(g), dim3(kFindBlock), 0, s,
                          t->template view<NW, VER>(), keys, vals, n, ts);
     else
-      hipLaunchKernelGGL((find_kernel<NW, VER, false>), dim3(g), dim3(kBlock), 0, s,
+      hipLaunchKernelGGL((find_kernel<NW, VER, false>), dim3(g), dim3(kFindBlock), 0, s,
                          t->template view<NW, VER>(), keys, vals, n, ts);
     MVBT_HIP(hipGetLastError());
     return MVBT_OK;

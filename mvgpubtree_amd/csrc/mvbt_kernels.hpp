@@ -9,6 +9,14 @@
 
 namespace mvbt {
 
+// The find kernel: 1024-thread workgroups sharing one large route cache. The
+// range kernel keeps 256-thread workgroups: at 4 waves per SIMD a CU then
+// holds one 1024-thread workgroup at a time and idles while its slowest wave
+// finishes (measured 1.5% slower with the large cache).
+constexpr int kFindBlock = 1024;
+constexpr uint32_t kFindCacheBytes = 72 * 1024;
+constexpr uint32_t kRangeCacheBytes = 16 * 1024;
+
 template <int W>
 struct chunking {
   uint64_t first, stride;
@@ -45,11 +53,11 @@ __global__ void init_kernel(tree_view<NW, VER> tv) {
 }
 
 template <int NW, bool VER, bool CONC>
-__global__ void __launch_bounds__(256) find_kernel(tree_view<NW, VER> tv, const uint32_t* keys,
-                                                   uint32_t* values, uint64_t n, uint32_t ts) {
+__global__ void __launch_bounds__(kFindBlock) find_kernel(tree_view<NW, VER> tv, const uint32_t* keys,
+                                                          uint32_t* values, uint64_t n, uint32_t ts) {
   tile<NW> t;
   chunking<NW> c;
-  __shared__ route_cache<NW> rc;
+  __shared__ route_cache<NW, kFindCacheBytes> rc;
   tv.build_route_cache(rc);
   for (uint64_t base = c.first; base < n; base += c.stride) {
     uint64_t i = base + t.rank();
@@ -78,7 +86,7 @@ __global__ void __launch_bounds__(256)
                  uint64_t* out, uint32_t* counts, uint32_t L, uint64_t n, uint32_t ts) {
   tile<NW> t;
   chunking<NW> c;
-  __shared__ route_cache<NW> rc;
+  __shared__ route_cache<NW, kRangeCacheBytes> rc;
   tv.build_route_cache(rc);
   for (uint64_t base = c.first; base < n; base += c.stride) {
     uint64_t i = base + t.rank();

@@ -112,12 +112,14 @@ struct tree_ptrs {
 // cached node occupy consecutive slots in entry order. An image may go stale
 // while the kernel runs (a concurrent writer's splits); it then routes to a
 // node left of the right one, which the B-link right moves of the first
-// uncached level repair. At 40M keys and 16-pair nodes the cache holds the
-// root and the two levels below it (about 45 nodes).
-
-template <int NW>
+// uncached level repair. The find kernel runs 1024-thread workgroups so that
+// two of them (32 waves) share a CU with 72 KB of node images each: at 40M
+// keys and 16-pair nodes that is the root and the four levels below it
+// (about 520 nodes). The range kernel (4 waves per SIMD) keeps 256-thread
+// workgroups, whose 16 KB hold the root and two levels (about 54 nodes).
+template <int NW, uint32_t BYTES>
 struct route_cache {
-  static constexpr uint32_t kBytes = 16384;
+  static constexpr uint32_t kBytes = BYTES;
   static constexpr uint32_t kMaxNodes = kBytes / (NW * 8);
   static constexpr uint16_t kNoChild = 0xFFFF;
   uint64_t nodes[kMaxNodes * NW];
@@ -494,9 +496,9 @@ struct tree_view : tree_ptrs {
   // key in lane j go to out + j * cap (at most cap of them; out may be null)
   // and the true count is returned. Keys outside `todo` return kEmpty / 0.
   static constexpr int kReadGroup = NW < 8 ? NW : 8;
-  template <bool RANGE>
+  template <bool RANGE, class RC = route_cache<NW, 1024>>
   MVBT_DEV uint32_t read_chunk(uint32_t lo, uint32_t hi, uint32_t ts, uint64_t* out, uint32_t cap,
-                               uint64_t todo, const tile_t& t, const route_cache<NW>* rc = nullptr) const {
+                               uint64_t todo, const tile_t& t, const RC* rc = nullptr) const {
     enum : uint32_t { kRoute = 0, kArrive = 1, kChain = 2 };
     const uint32_t r = t.rank();
     uint32_t cur = root_, phase = kRoute, cnt = 0;
@@ -681,8 +683,9 @@ struct tree_view : tree_ptrs {
 
   // Block-wide (every thread of the workgroup calls it, before any early
   // return; blockDim.x >= 64): copies the top levels into `rc`.
-  MVBT_DEV void build_route_cache(route_cache<NW>& rc) const {
-    using rc_t = route_cache<NW>;
+  template <class RC>
+  MVBT_DEV void build_route_cache(RC& rc) const {
+    using rc_t = RC;
     const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
     if (tid == 0) {
       rc.index[0] = root_;
@@ -754,8 +757,9 @@ struct tree_view : tree_ptrs {
   // read from global memory and, in `parent`, the node it was reached from
   // (kNone after a right move). A torn or stale image only routes left of the
   // right node; the root is returned if an image names no node of the pool.
-  MVBT_DEV uint32_t route_lane(const route_cache<NW>& rc, uint32_t key, uint32_t& parent) const {
-    using rc_t = route_cache<NW>;
+  template <class RC>
+  MVBT_DEV uint32_t route_lane(const RC& rc, uint32_t key, uint32_t& parent) const {
+    using rc_t = RC;
     uint32_t s = 0;
     parent = kNone;
     for (uint32_t step = 0; step < 2 * rc_t::kMaxNodes; ++step) {
