@@ -368,8 +368,8 @@ def run_config4(args):
     modes = tuple(int(x) for x in args.modes.split(","))
     ncfg = len(waves) * len(modes)
     if len(widths) * ncfg > 1:
-        tw = wl.config4(seed=44, n_initial=args.keys, num_batches=2 * ncfg * args.tune_batches)
-        groups = [tw.batches[i::2 * ncfg] for i in range(2 * ncfg)]
+        tw = wl.config4(seed=44, n_initial=args.keys, num_batches=2 * args.tune_batches)
+        groups = [tw.batches[0::2], tw.batches[1::2]]  # (warm, timed), the same for every configuration
         best, allr = autotune.tune_batched(tw.initial_keys, tw.initial_values, groups, L, dev,
                                            node_widths=widths, waves=waves, log=log, modes=modes)
         del tw
@@ -572,7 +572,8 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="config 5: requests per QPS step")
     ap.add_argument("--step-ops", type=int, default=1 << 20, help="config 5: max ops per served step")
     ap.add_argument("--batches", type=int, default=4096, help="config 4: number of batches")
-    ap.add_argument("--tune-batches", type=int, default=48, help="config 4: batches per tuning group")
+    ap.add_argument("--tune-batches", type=int, default=192,
+                    help="config 4: batches in the timed tuning group (and in the warm-up group)")
     ap.add_argument("--modes", default="0,1", help="config 4: worker modes to tune (0 one op per tile, 1 runs)")
     ap.add_argument("--widths", default="16,32,64", help="config 4: node widths to tune")
     ap.add_argument("--waves", default="8,16,24,32", help="config 4: waves/CU to tune")

@@ -38,20 +38,21 @@ def tune_batched(initial_keys, initial_values, groups, L: int, dev,
                  log=None, modes=(1,)):
     """Returns (best TuneResult, all results).
 
-    `groups` holds 2 * len(waves) * len(modes) disjoint lists of batches cut
-    from one workload (fresh insert keys and distinct erase keys throughout):
-    for configuration i (a mode and a wave count), group 2i warms it up (graph
-    build for that grid, code load) and group 2i + 1 is timed. Every node width
-    gets its own tree built from the full initial key set, so each
-    configuration starts from the same tree."""
+    `groups` is (warm, timed): two disjoint lists of batches cut from one
+    workload. Every configuration (node width, worker mode, waves per CU)
+    gets a tree freshly built from the full initial key set, runs `warm`
+    (graph build for that grid, code load) and then `timed`, which is timed:
+    all configurations do exactly the same work from the same start. (Timing
+    each configuration on its own slice of a workload with log-uniform batch
+    sizes let the slices' different mixes decide the winner.)"""
     import torch
 
     import mvgpubtree_amd as mv
 
     combos = [(m, wv) for m in modes for wv in waves]
-    if len(groups) != 2 * len(combos):
-        raise ValueError("need 2 * len(waves) * len(modes) batch groups")
-    key = f"{_device_name()}|{initial_keys.size}|{[len(g) for g in groups]}|{L}|{list(modes)}"
+    if len(groups) != 2:
+        raise ValueError("groups must be (warm batches, timed batches)")
+    key = f"{_device_name()}|{initial_keys.size}|{[len(g) for g in groups]}|{L}|{list(modes)}|same"
     if cache and os.path.exists(cache):
         try:
             data = json.load(open(cache))
@@ -60,34 +61,32 @@ def tune_batched(initial_keys, initial_values, groups, L: int, dev,
                 return best, [TuneResult(**r) for r in data[key]["all"]]
         except (OSError, ValueError, TypeError):
             pass
-    runners = [drivers.BatchedRunner(g, L, dev, seed=i) for i, g in enumerate(groups)]
-    total_ops = sum(r.nops for r in runners)
+    warm, timed = (drivers.BatchedRunner(g, L, dev, seed=i) for i, g in enumerate(groups))
+    total_ops = warm.nops + timed.nops
     dk = drivers._dev(initial_keys, dev)
     dv = drivers._dev(initial_values, dev)
     results = []
     for nw in node_widths:
         pool = int(initial_keys.size / 4 * (16 / nw) + 2 * total_ops) + (1 << 20)
-        tree = mv.GpuVersionedBTree(pool_nodes=pool, node_width=nw, retire_capacity=1 << 24)
-        tree.insert(dk, dv)
-        for i, (mode, wv) in enumerate(combos):
+        for mode, wv in combos:
+            tree = mv.GpuVersionedBTree(pool_nodes=pool, node_width=nw, retire_capacity=1 << 24)
+            tree.insert(dk, dv)
             tree.set_worker_mode(mode)
             tree.set_worker_waves(wv)
-            runners[2 * i].run(tree, use_graph=True)
+            warm.run(tree, use_graph=True)
             tree.reclaim()
-            runner = runners[2 * i + 1]
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            runner.run(tree, use_graph=True)
+            timed.run(tree, use_graph=True)
             torch.cuda.synchronize()
             ms = (time.perf_counter() - t0) * 1e3
-            tree.reclaim()
-            r = TuneResult(nw, wv, ms, runner.nops / ms / 1e3, mode)
+            if tree.error_flags():
+                raise RuntimeError(f"autotune: device error flags 0x{tree.error_flags():x}")
+            tree.close()
+            r = TuneResult(nw, wv, ms, timed.nops / ms / 1e3, mode)
             results.append(r)
             if log:
                 log(f"autotune nw={nw} mode={mode} waves/CU={wv}: {ms:.2f} ms ({r.mops:.1f} M ops/s)")
-        if tree.error_flags():
-            raise RuntimeError(f"autotune: device error flags 0x{tree.error_flags():x}")
-        tree.close()
     best = max(results, key=lambda r: r.mops)
     if cache:
         data = {}
