@@ -1213,8 +1213,8 @@ struct tree_view : tree_ptrs {
   template <int OP>
   MVBT_DEV uint64_t write_chunk(uint32_t key, uint32_t value, uint64_t q, uint32_t leaf,
                                 uint32_t& result, write_ctx& c, const tile_t& t) const {
-    if constexpr (NW > 16) {
-      return q;  // wide nodes: register budget; per-key path
+    if constexpr (NW > 32) {
+      return q;  // 64-pair nodes: register budget; per-key path
     } else {
       const uint32_t r = t.rank();
       const uint64_t todo = q & t.ballot(valid_key(key) && leaf < cap_);
