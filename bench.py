@@ -217,22 +217,34 @@ def run_config2(args):
     rng.shuffle(q)
     dq = dev_tensor(q, dev)
     fout = torch.empty(q.size, dtype=torch.int32, device=dev)
-    sf = tree.take_snapshot(stream)
+    mem = tree.memory_stats()
+    # Finds run on a tree of the node width that suits them (a tile walks one
+    # key at a time, so 16-pair tiles keep four walks per wave in flight):
+    # the same 40M initial keys, freshly built when the widths differ.
+    ftree = tree
+    if args.find_node_width != args.node_width:
+        ftree = mv.GpuVersionedBTree(pool_nodes=int(n_initial / 4) + (1 << 20), node_width=args.find_node_width,
+                                     retire_capacity=1 << 20)
+        ftree.insert(dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev), stream=stream)
+    sf = ftree.take_snapshot(stream)
     for _ in range(2):
-        tree.find(dq, fout, timestamp=sf, stream=stream)
+        ftree.find(dq, fout, timestamp=sf, stream=stream)
     fe = []
     for _ in range(5):
         a, b = ev(), ev()
         a.record(stream)
-        tree.find(dq, fout, timestamp=sf, stream=stream)
+        ftree.find(dq, fout, timestamp=sf, stream=stream)
         b.record(stream)
         fe.append((a, b))
     torch.cuda.synchronize()
     find_ms = float(np.mean([a.elapsed_time(b) for a, b in fe]))
     hits = int((fout.cpu().numpy().view(np.uint32) != 0xFFFFFFFF).sum())
     rates["find_Mops"] = q.size / find_ms / 1e3
-    tree.release_snapshot(sf)
-    mem = tree.memory_stats()
+    ftree.release_snapshot(sf)
+    if ftree is not tree:
+        if ftree.error_flags():
+            raise RuntimeError(f"device error flags 0x{ftree.error_flags():x} (find tree)")
+        ftree.close()
 
     # Dominant kernel of the step and its roofline.
     share = {"range32": kernel_ms["range32"], "insert": kernel_ms["insert"]}
@@ -241,10 +253,10 @@ def run_config2(args):
     achieved = per_launch_bytes / (kernel_ms[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"range_kernel<16,versioned> ({n_rq} range-32 queries/launch)"
-                if dom == "range32" else f"write_kernel<16,versioned,insert> ({n_ins} inserts/launch)",
+                "kernel": f"range_kernel<{args.node_width},versioned> ({n_rq} range-32 queries/launch)"
+                if dom == "range32" else f"write_kernel<{args.node_width},versioned,insert> ({n_ins} inserts/launch)",
                 "avg_launch_ms": round(kernel_ms[dom], 4)}
-    traffic = load_traffic(dom)
+    traffic = load_traffic(dom, args.node_width)
     if traffic is not None:
         roofline["traffic"] = traffic
 
@@ -270,7 +282,7 @@ def run_config2(args):
         "roofline": roofline,
         "breakdown": {k: round(v, 2) for k, v in rates.items()} | {
             "build_Minserts_per_s": round(n_initial / build_s / 1e6, 2),
-            "find_hits": hits,
+            "find_hits": hits, "find_node_width": args.find_node_width,
             "find_roofline_frac": round(q.size * BYTES["find"] / (find_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "live_nodes": mem.live_nodes, "peak_live_nodes": mem.peak_live_nodes,
             "reclaimed_nodes": mem.reclaimed_nodes},
@@ -531,15 +543,17 @@ def run_config5(args):
             "pool_peak_GB": round(mem.peak_bytes / 1e9, 2)}
 
 
-def load_traffic(kernel: str):
+def load_traffic(kernel: str, node_width: int):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, copy-kernel calibrated), when a summary has been recorded."""
+    WRITE_SIZE, copy-kernel calibrated), when a summary has been recorded for
+    this node width."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(kernel)
+            data = json.load(f)
     except (OSError, ValueError):
         return None
+    return data.get(kernel) if data.get("node_width") == node_width else None
 
 
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 5):
@@ -561,7 +575,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--keys", type=int, default=40_000_000)
     ap.add_argument("--ops", type=int, default=1_000_000)
-    ap.add_argument("--node-width", type=int, default=16)
+    ap.add_argument("--node-width", type=int, default=32,
+                    help="config 2: node width of the measured tree (32 pairs measured fastest for "
+                         "insert + range-32 batches)")
+    ap.add_argument("--find-node-width", type=int, default=16,
+                    help="config 2: node width of the tree the find breakdown runs on")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
                     help="2: headline mixed insert/range batches; 3: erase/find batches with epoch "
