@@ -693,6 +693,9 @@ mvbt_status mvbt_take_snapshot(mvbt_tree* t, void* stream, uint32_t* snapshot_id
   if (!t->external_writers && slot_free) {
     const uint32_t id = t->clock;
     if (id == MVBT_LATEST) return fail(MVBT_ERR_UNSUPPORTED, "snapshot ids exhausted");
+    // a reclamation launched earlier (on any stream) may still have to read
+    // the edge a released snapshot left in this slot
+    if (t->maint_ev) MVBT_HIP(hipStreamWaitEvent(s, t->maint_ev, 0));
     hipLaunchKernelGGL(snapshot_kernel, dim3(1), dim3(64), 0, s, t->st, nullptr, t->snap_pos + slot, id);
     MVBT_HIP(hipGetLastError());
     if (mvbt_status rc = clock_advanced(t, s, 1)) return rc;
