@@ -89,7 +89,7 @@ def test_write_on_other_stream_after_snapshot_is_invisible(dev):
         r = to_np(got)
         want_base = wl.value_of(base[:20_000])
         assert np.array_equal(r[:20_000], want_base)
-        # keys of this round's write are absent at s; earlier rounds' are present
+        # keys written in this pass are absent at s; those of earlier passes are present
         lk = late[:20_000]
         present = np.isin(lk, late[:rnd * 30_000])
         want = np.where(present, wl.value_of(lk), np.uint32(mv.NOT_FOUND))
