@@ -47,6 +47,20 @@ def test_header_constants_match_mirror():
     assert int(consts["MVBT_FLAG_RETIRE_OVERFLOW"], 16) == mv.FLAG_RETIRE_OVERFLOW
     assert int(consts["MVBT_FLAG_INVALID_KEY"], 16) == mv.FLAG_INVALID_KEY
     assert int(consts["MVBT_FLAG_VERSION_GAP"], 16) == mv.FLAG_VERSION_GAP
+    assert int(consts["MVBT_FLAG_RANGE_OVERFLOW"], 16) == mv.FLAG_RANGE_OVERFLOW
+    assert int(consts["MVBT_FLAG_CORRUPT"], 16) == mv.FLAG_CORRUPT
+    assert int(consts["MVBT_FLAG_SNAPSHOT_ID"], 16) == mv.FLAG_SNAPSHOT_ID
+
+
+def test_device_error_bits_match_header_flags():
+    """The kernels raise kErr* bits (csrc/mvbt_common.hpp); the ABI documents
+    them as MVBT_FLAG_* (include/mvbt.h): the two lists must agree bit for bit."""
+    text = open(HEADER).read()
+    flags = {k: int(v, 16) for k, v in re.findall(r"#define\s+MVBT_FLAG_(\w+)\s+(0x[0-9A-Fa-f]+)u", text)}
+    common = open(os.path.join(ROOT, "mvgpubtree_amd", "csrc", "mvbt_common.hpp")).read()
+    bits = {k: 1 << int(v) for k, v in re.findall(r"kErr(\w+)\s*=\s*1u\s*<<\s*(\d+)", common)}
+    snake = {re.sub(r"(?<!^)(?=[A-Z])", "_", k).upper(): v for k, v in bits.items()}
+    assert snake == flags
 
 
 def test_null_handle_is_an_error_not_a_crash():
