@@ -1198,17 +1198,21 @@ struct tree_view : tree_ptrs {
   }
 
   // Batched leaf phase for a routed chunk (lane j: key j, value j, routed
-  // leaf j; `q` the keys to write). The leaves that receive the chunk's keys
-  // are handled kLeafBatch at a time, each step for all of them before the
-  // next: lock (one lane-parallel atomic), reload, apply the keys in
-  // registers in lane order, store the version copies, one wait, store the
-  // version words (lane-parallel), one wait, store the data lanes, one wait,
-  // unlock (lane-parallel). Per leaf this is the order write() uses
-  // (copy -> version word -> data -> unlock), so snapshot readers validate
-  // the same way; a chunk pays about six waits per batch of leaves instead
-  // of six per key. Keys whose leaf was not locked, would overflow, or no
-  // longer covers the key are returned as a mask for write(); lane j's
-  // `result` receives kWriteNew / kWriteSame for the others.
+  // leaf j; `q` the keys to write). Every leaf that receives the chunk's keys
+  // is locked at once (one lane-parallel atomic). The locked leaves are then
+  // handled kLeafBatch at a time, each step for all of them before the next:
+  // reload, apply the keys in registers in lane order, store the version
+  // copies, one wait, store the version words (lane-parallel), one wait,
+  // store the data lanes; the next batch's reloads are issued before the wait
+  // that completes those stores, and then this batch is unlocked
+  // (lane-parallel). Per leaf this is the order write() uses (copy -> version
+  // word -> data -> unlock), so snapshot readers validate the same way; a
+  // chunk pays about three waits per batch of leaves instead of six per key.
+  // No lock is waited for here and every lock is released before returning,
+  // so the tiles of a wave never wait on each other's locks inside it. Keys
+  // whose leaf was not locked, would overflow, or no longer covers the key
+  // are returned as a mask for write(); lane j's `result` receives
+  // kWriteNew / kWriteSame for the others.
   static constexpr int kLeafBatch = 8;
   template <int OP>
   MVBT_DEV uint64_t write_chunk(uint32_t key, uint32_t value, uint64_t q, uint32_t leaf,
@@ -1229,23 +1233,24 @@ struct tree_view : tree_ptrs {
         if ((same >> r) & 1) my_lead = j;
         m &= ~same;
       }
-      for (uint64_t pending = lead; pending;) {
-        uint64_t batch = 0;
+      // 1. lock every leaf of the chunk at once (each leader lane its own leaf)
+      uint32_t old = kLockBit;
+      if ((lead >> r) & 1) old = atomic_or32(lock_word(leaf), kLockBit);
+      const uint64_t locked = lead & t.ballot(!(old & kLockBit));
+      for (uint64_t m = lead & ~locked; m; m &= m - 1)
+        left |= todo & t.ballot(my_lead == (uint32_t)lowest_bit(m));
+      uint64_t pending = locked, prev = 0;
+      for (;;) {
+        uint64_t got = 0;
         {
           uint64_t m = pending;
           for (int i = 0; i < kLeafBatch && m; ++i) {
-            batch |= m & (~m + 1);
+            got |= m & (~m + 1);
             m &= m - 1;
           }
         }
-        pending &= ~batch;
-        // 1. lock every leaf of the batch (each leader lane its own leaf)
-        uint32_t old = kLockBit;
-        if ((batch >> r) & 1) old = atomic_or32(lock_word(leaf), kLockBit);
-        const uint64_t got = batch & t.ballot(!(old & kLockBit));
-        for (uint64_t m = batch & ~got; m; m &= m - 1)
-          left |= todo & t.ballot(my_lead == (uint32_t)lowest_bit(m));
-        // 2. reload the locked leaves
+        pending &= ~got;
+        // 2. reload this batch's leaves
         uint64_t img[kLeafBatch];
         uint32_t gl[kLeafBatch];
         {
@@ -1257,6 +1262,13 @@ struct tree_view : tree_ptrs {
             m &= m - 1;
           }
         }
+        // 6 (of the previous batch). unlock its leaves once its data stores
+        // are complete: that wait also covers the reloads just issued
+        if (prev) {
+          order_release();
+          if ((prev >> r) & 1) atomic_and32(lock_word(leaf), ~kLockBit);
+        }
+        if (!got) break;
         const int ng = __builtin_popcountll(got);
         // 3. plan: apply each leaf's keys in registers; a leaf whose keys do
         // not all fit (or that no longer covers one of them) goes per key
@@ -1348,12 +1360,10 @@ struct tree_view : tree_ptrs {
           }
           st_coherent(at(target) + r, n.p);
         }
-        // 6. unlock every locked leaf
-        order_release();
-        if ((got >> r) & 1) atomic_and32(lock_word(leaf), ~kLockBit);
         // keys of leaves that failed for lack of memory go per key (they fail there too)
         for (int i = 0; i < ng; ++i)
           if (!((ok >> i) & 1)) left |= todo & t.ballot(my_lead == gl[i]);
+        prev = got;
       }
       return left;
     }
