@@ -1213,7 +1213,9 @@ struct tree_view : tree_ptrs {
   // whose leaf was not locked, would overflow, or no longer covers the key
   // are returned as a mask for write(); lane j's `result` receives
   // kWriteNew / kWriteSame for the others.
-  static constexpr int kLeafBatch = 8;
+  // 32-pair nodes take 32-key chunks: batches of 12 leaves cover them in
+  // three rounds within the same register budget (4 waves per SIMD).
+  static constexpr int kLeafBatch = NW == 32 ? 12 : 8;
   template <int OP>
   MVBT_DEV uint64_t write_chunk(uint32_t key, uint32_t value, uint64_t q, uint32_t leaf,
                                 uint32_t& result, write_ctx& c, const tile_t& t) const {
