@@ -52,8 +52,9 @@ def build_lib(verbose: bool = False) -> str:
 def build_example(verbose: bool = False) -> str:
     """C++ host program over include/gpu_btree.h, linked against libmvbt.so."""
     src = os.path.join(CSRC, "capi_example.cpp")
-    cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", f"-I{os.path.join(ROOT, 'include')}", "-o", EXAMPLE, src,
-           f"-L{PKG}", "-lmvbt", f"-Wl,-rpath,{PKG}"]
+    cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-Wno-unused-command-line-argument", f"-I{os.path.join(ROOT, 'include')}", "-o",
+           EXAMPLE, src, f"-L{PKG}", "-lmvbt", f"-Wl,-rpath,{PKG}"]
     _run(cmd, verbose)
     return EXAMPLE
 
