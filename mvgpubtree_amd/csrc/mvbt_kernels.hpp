@@ -120,8 +120,16 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
                                                     const uint32_t* values, uint64_t n) {
   tile<NW> t;
   chunking<NW> c;
-  // No route cache here: at one chunk per tile the block-wide copy of the
-  // top levels delays every tile by more than the levels it saves.
+  using node_t = node<NW, VER>;
+  // The top levels in LDS, at 32-pair nodes only: a tile routes its 32 keys
+  // in four groups, and the cache saves each group the top passes. At
+  // 16-pair nodes (two groups, one chunk per tile) the copy delayed every
+  // tile by more than it saved. The images go stale as this launch's splits
+  // land, so the cache is used only when the tree's nodes can hold four
+  // times the batch (a stale image then costs a right move or two); a
+  // launch that grows a small tree routes from the root.
+  __shared__ route_cache<NW, 16384> rc;
+  if constexpr (NW == 32) tv.build_route_cache(rc, 4 * n / node_t::kData + 1);
   if (c.first >= n) return;
   write_ctx ctx = tv.begin_writes(t);
   ctx.exact = 1;  // ctx.want below is the chunk's exact remaining count
@@ -135,7 +143,11 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
     // Route the whole chunk first (one latency per level for all its keys),
     // then write key by key starting from the routed leaf.
     uint32_t leaf = kNone, par = kNone;
-    tv.template descend_chunk<true>(key, q, leaf, par, t, q);
+    uint32_t st0 = kNone, sp0 = kNone;
+    if constexpr (NW == 32) {
+      if (active && valid_key(key)) st0 = tv.route_lane(rc, key, sp0);
+    }
+    tv.template descend_chunk<true>(key, q, leaf, par, t, q, st0, sp0);
     // Leaves that take keys of this chunk are written together; the keys it
     // hands back (lock busy, split needed, leaf moved) go one at a time.
     // ctx.want is the chunk's exact count: refills for version copies take

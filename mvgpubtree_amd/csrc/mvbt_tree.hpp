@@ -104,10 +104,11 @@ struct tree_ptrs {
   uint32_t cap_;           // pool capacity: every followed index must be below it
 };
 
-// The top levels of the tree, copied into LDS by each workgroup of a read
-// kernel when it starts (tree_view::build_route_cache) and used for routing
-// only (tree_view::route_lane): a find or range query then reads from global
-// memory only the levels below them. Levels are cached breadth-first, whole,
+// The top levels of the tree, copied into LDS by each workgroup of the find
+// and range kernels (and of the insert kernel at 32-pair nodes) when it
+// starts (tree_view::build_route_cache) and used for routing only
+// (tree_view::route_lane): an operation then reads from global memory only
+// the levels below them. Levels are cached breadth-first, whole,
 // while they hold internal nodes only and fit kBytes; the children of a
 // cached node occupy consecutive slots in entry order. An image may go stale
 // while the kernel runs (a concurrent writer's splits); it then routes to a
@@ -420,6 +421,8 @@ struct tree_view : tree_ptrs {
   // its parent (kNone when the leaf was reached by a right move), or kNone as
   // leaf when the walk left the pool (kErrCorrupt is flagged). Routing only:
   // writers re-check both under lock, readers validate leaf versions.
+  // `start` / `start_par` (per lane): where lane j's walk begins (a node from
+  // the LDS route cache) and the node it was reached from; kNone: the root.
   // Lanes in `early` (writes) stop at the level above the leaves and take the
   // leaf from their parent's entry without reading it: a writer locks the
   // leaf before it reads it anyway, and checks under the lock that it is a
@@ -429,9 +432,14 @@ struct tree_view : tree_ptrs {
   static constexpr int kGroup = NW < 8 ? NW : 8;
   template <bool CONC>
   MVBT_DEV void descend_chunk(uint32_t key, uint64_t todo, uint32_t& leaf, uint32_t& parent,
-                              const tile_t& t, uint64_t early = 0) const {
+                              const tile_t& t, uint64_t early = 0, uint32_t start = kNone,
+                              uint32_t start_par = kNone) const {
     const uint32_t r = t.rank();
     uint32_t cur = root_, par = kNone;
+    if (start < cap_) {
+      cur = start;
+      par = start_par;
+    }
     bool corrupt = false;
     todo &= t.ballot(valid_key(key));  // reserved keys are not routed (callers reject them)
     for (int base = 0; base < NW; base += kGroup) {
@@ -682,17 +690,21 @@ struct tree_view : tree_ptrs {
   }
 
   // Block-wide (every thread of the workgroup calls it, before any early
-  // return; blockDim.x >= 64): copies the top levels into `rc`.
+  // return; blockDim.x >= 64): copies the top levels into `rc`. The cache
+  // stays empty (routing starts at the root) while fewer than `min_nodes`
+  // nodes have been handed out.
   template <class RC>
-  MVBT_DEV void build_route_cache(RC& rc) const {
+  MVBT_DEV void build_route_cache(RC& rc, unsigned long long min_nodes = 0) const {
     using rc_t = RC;
     const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
     if (tid == 0) {
+      const bool use = min_nodes == 0 || ld_coherent((const uint64_t*)&st->bump) >= min_nodes;
       rc.index[0] = root_;
-      rc.count = 1;
-      rc.stop = 0;
+      rc.count = use ? 1u : 0u;
+      rc.stop = use ? 0u : 1u;
     }
     __syncthreads();
+    if (rc.stop) return;
     uint32_t b = 0, e = 1;
     // every level that does not stop the build adds at least one slot, so
     // the build stops (rc.stop) within kMaxNodes levels
@@ -762,6 +774,7 @@ struct tree_view : tree_ptrs {
     using rc_t = RC;
     uint32_t s = 0;
     parent = kNone;
+    if (rc.count == 0) return root_;
     for (uint32_t step = 0; step < 2 * rc_t::kMaxNodes; ++step) {
       const uint64_t* img = rc.nodes + s * NW;
       const uint64_t m = img[node_t::kMetaLane];
