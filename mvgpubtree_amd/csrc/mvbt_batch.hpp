@@ -317,6 +317,14 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
   int32_t* const my_frontier = job->frontier + (wave % kFrontierCopies) * 32;
   const unsigned long long tiles_total = (unsigned long long)gridDim.x * (blockDim.x / NW);
   uint32_t backoff = 1;
+  // The top levels in LDS (as in write_kernel, at 32-pair nodes only), copied
+  // once per block of the persistent launch: used when the job gives every
+  // tile several operations (the copy delays the block's first one) and the
+  // tree's nodes can hold the whole job (its splits then leave the images
+  // about one right move stale at most).
+  __shared__ route_cache<NW, 16384> rc;
+  if constexpr (NW == 32)
+    tv.build_route_cache(rc, nops >= 4 * tiles_total ? nops / node_t::kData + 1 : ~0ull);
   write_ctx ctx = tv.begin_writes(t);
   ctx.exact = 1;
   int32_t seen = -1;
@@ -382,7 +390,11 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
     if (op.type == kOpRange && rkey == 0) rkey = 1;
     const bool route = mine && valid_key(rkey) && !(op.type == kOpRange && op.arg <= rkey);
     uint32_t leaf = kNone, par = kNone, res = 0;
-    tv.template descend_chunk<true>(rkey, t.ballot(route), leaf, par, t, t.ballot(is_w));
+    uint32_t st0 = kNone, sp0 = kNone;
+    if constexpr (NW == 32) {
+      if (route) st0 = tv.route_lane(rc, rkey, sp0);
+    }
+    tv.template descend_chunk<true>(rkey, t.ballot(route), leaf, par, t, t.ballot(is_w), st0, sp0);
     // writes: batched leaf phase, then the keys it hands back one by one
     ctx.cur = clock0 + b + 1;
     const uint64_t qw = t.ballot(is_w);

@@ -105,7 +105,8 @@ struct tree_ptrs {
 };
 
 // The top levels of the tree, copied into LDS by each workgroup of the find
-// and range kernels (and of the insert kernel at 32-pair nodes) when it
+// and range kernels (and of the insert kernel and the chunked batched worker
+// at 32-pair nodes) when it
 // starts (tree_view::build_route_cache) and used for routing only
 // (tree_view::route_lane): an operation then reads from global memory only
 // the levels below them. Levels are cached breadth-first, whole,
