@@ -346,3 +346,35 @@ def test_export_dot(dev, kind):
     small = tree.to_dot(max_nodes=3)
     assert small.count("label=") <= 3
     tree.close()
+
+
+@pytest.mark.parametrize("kind", ["plain", "versioned"])
+def test_concurrent_writers_on_two_streams_route_through_stale_caches(dev, kind):
+    """32-pair nodes, a tree large enough that the insert launch routes
+    through its LDS copy of the top levels: two streams insert disjoint key
+    sets at once, so each launch's images go stale under the other's splits
+    (they must route left and repair by right moves). The result must equal
+    the oracle and the structure must validate."""
+    import torch
+    rng = np.random.default_rng(21)
+    keys = wl.unique_keys(rng, 700_000)
+    base, a, b = keys[:300_000], keys[300_000:500_000], keys[500_000:]
+    cls = mv.GpuVersionedBTree if kind == "versioned" else mv.GpuBlinkTree
+    tree = cls(pool_nodes=1 << 17, node_width=32)
+    tree.insert(to_dev(base, dev), to_dev(wl.value_of(base), dev))
+    sync()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    da, dva = to_dev(a, dev), to_dev(wl.value_of(a), dev)
+    db, dvb = to_dev(b, dev), to_dev(wl.value_of(b), dev)
+    for part in range(4):  # batches of 50K: small against the tree, so the cache is used
+        sl = slice(part * 50_000, (part + 1) * 50_000)
+        tree.insert(da[sl], dva[sl], stream=s1)
+        tree.insert(db[sl], dvb[sl], stream=s2)
+    sync()
+    ref = oracle.SortedState(keys, wl.value_of(keys))
+    assert tree.validate_tree_structure()[0] == keys.size
+    assert np.array_equal(tree.export(), (ref.keys.astype(np.uint64) | (ref.values.astype(np.uint64) << 32)))
+    q = np.concatenate([keys[::7], wl.absent_keys(rng, 20_000, np.sort(keys))])
+    assert np.array_equal(to_np(tree.find(to_dev(q, dev))), ref.find(q))
+    assert_clean(tree)
+    tree.close()
