@@ -54,13 +54,16 @@ def test_batched_form_matches_oracle(dev, nw, use_graph, mode):
     tree.close()
 
 
+@pytest.mark.parametrize("nw", [16, 32])
 @pytest.mark.parametrize("waves", [1, 2])
-def test_chunked_worker_multi_op_chunks(dev, waves):
-    """Few worker waves and large batches, so tiles claim runs of up to 16
-    operations that span batch boundaries (batch size / tiles in the grid)."""
+def test_chunked_worker_multi_op_chunks(dev, waves, nw):
+    """Few worker waves and large batches, so tiles claim runs of up to NW
+    operations that span batch boundaries (batch size / tiles in the grid).
+    At 32-pair nodes the worker also routes through its LDS copy of the top
+    levels, which the job's own splits make stale."""
     w = wl.config4(seed=77 + waves, n_initial=300_000, num_batches=40, min_batch=32, max_batch=65536,
                    range_capacity=256)
-    tree = mv.GpuVersionedBTree(pool_nodes=1 << 20, node_width=16)
+    tree = mv.GpuVersionedBTree(pool_nodes=1 << 20, node_width=nw)
     tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
     tree.set_worker_mode(1)
     tree.set_worker_waves(waves)
