@@ -1,7 +1,8 @@
 """Benchmark: M ops/s of the multiversion B-link tree on one MI355X.
 
-Default workload (the metric's configuration): versioned tree, uint32 keys and
-values, bulk insert of 40M unique uniform keys, then batches of 1M ops each:
+Default workload (the metric's configuration): versioned tree with 32-pair
+nodes, uint32 keys and values, bulk insert of 40M unique uniform keys, then
+batches of 1M ops each:
 500K inserts of new keys and 500K range queries [lo, lo + span) holding ~32
 keys (result slots L = 128), every range reading the snapshot taken before its
 batch. One step = one batch: take_snapshot, range_query at it, insert,
@@ -10,10 +11,11 @@ release the snapshot, reclaim. Inputs are resident in HBM before timing.
 Prints one JSON line with the contract fields plus:
   roofline      dominant kernel's algorithmic bytes / average launch time
                 against 8 TB/s (HIP events on the launch stream);
-  cpu_baseline  the oracle (single-thread std::map restatement of the
-                reference's serial validation path) on one batch of the same
-                workload on this host;
-  breakdown     per-operation rates, plus 1M finds (50% hits) on the same tree.
+  cpu_baseline  the oracle (std::map restatement of the reference's serial
+                validation path) on one batch of the same workload on this
+                host: one thread, and key-range shards over the host's cores;
+  breakdown     per-operation rates, plus 1M finds (50% hits) on a tree of the
+                same 40M keys with 16-pair nodes (--find-node-width).
 """
 from __future__ import annotations
 
