@@ -10,6 +10,12 @@ snapshot rules of include/gpu_versioned_blink_tree.hpp:297-353,1578-1603):
 * ``SortedState`` (``sorted_ref.py``): numpy sorted-array state for batch
   workloads at full size (tens of millions of keys), where every read in a
   batch sees the state before the batch.
+* ``replay_batches``: exact replay of a stream of mixed-operation batches on
+  the multiversion map (one snapshot before each batch), the semantics the
+  batched and served forms claim.
+* ``WriteOnceReplay``: the same results for streams in which no key is
+  written twice (the benchmark streams), answered op by op on several
+  threads from the sorted initial pairs; used at full size (40M-500M keys).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
 leg may import this package.
@@ -36,7 +42,7 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 def build(verbose: bool = False) -> str:
     """Compile mvmap.cpp into liboracle.so (g++ -O2)."""
     src = os.path.join(HERE, "mvmap.cpp")
-    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", LIB, src]
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-pthread", "-o", LIB, src]
     if verbose:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)
@@ -67,6 +73,15 @@ def lib():
         L.oracle_size.restype = ctypes.c_uint64
         L.oracle_export.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.c_uint32]
         L.oracle_export.restype = ctypes.c_uint64
+        L.oracle_replay_batches.argtypes = [ctypes.c_void_p, _u32p, ctypes.c_uint64, _u64p, ctypes.c_uint32,
+                                            _u32p, _u64p, ctypes.c_uint64, ctypes.c_uint32]
+        L.oracle_replay_batches.restype = ctypes.c_int64
+        L.oracle_wo_create.argtypes = [_u32p, _u32p, ctypes.c_uint64, _u32p, ctypes.c_uint64, _u64p,
+                                       ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32), _u64p]
+        L.oracle_wo_create.restype = ctypes.c_void_p
+        L.oracle_wo_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_wo_answer.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, _u32p, _u64p, ctypes.c_uint32,
+                                       ctypes.c_uint32]
         _lib = L
     return _lib
 
@@ -137,3 +152,79 @@ class MVMap:
         out = np.empty(n, dtype=np.uint64)
         lib().oracle_export(self._h, _p64(out), n, ts)
         return out
+
+
+class BatchConflict(ValueError):
+    """A batch writes one key twice, or names an unknown operation type."""
+
+    def __init__(self, op_index: int):
+        super().__init__(f"batch stream rejected at op {op_index}")
+        self.op_index = op_index
+
+
+def replay_batches(initial_keys, initial_values, ops, offsets, L: int, num_slots: int):
+    """Exact expected results of a batch stream (rows {type, key, arg, slot}):
+    batch by batch, a snapshot, every read at it, then the writes in op order.
+    Returns (results u32[n], pairs u64[num_slots * L] with unused slots
+    all-ones, MVMap holding the final state). Raises BatchConflict when a
+    batch writes one key twice."""
+    k = _u32(initial_keys)
+    order = np.argsort(k, kind="stable")
+    k, v = k[order], _u32(initial_values)[order]
+    last = np.ones(k.size, dtype=bool)
+    if k.size > 1:
+        last[:-1] = k[1:] != k[:-1]
+    m = MVMap()
+    m.build_sorted(np.ascontiguousarray(k[last]), np.ascontiguousarray(v[last]))
+    o = np.ascontiguousarray(ops, dtype=np.uint32).reshape(-1, 4)
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    res = np.zeros(o.shape[0], dtype=np.uint32)
+    pairs = np.full(max(num_slots, 0) * L, np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+    rc = lib().oracle_replay_batches(m._h, _p32(o), o.shape[0], _p64(off), off.size - 1, _p32(res),
+                                     _p64(pairs) if pairs.size else None, num_slots, L)
+    if rc < 0:
+        raise BatchConflict(-rc - 1)
+    return res, pairs, m
+
+
+class WriteOnceReplay:
+    """Expected results of a batch stream in which no key is written twice,
+    for any subset of its ops (each answer depends only on the initial pairs
+    and the stream's write events). `initial_keys` must be sorted, distinct
+    and free of reserved keys."""
+
+    def __init__(self, initial_keys, initial_values, ops, offsets):
+        self._k = _u32(initial_keys)
+        self._v = _u32(initial_values)
+        self._ops = np.ascontiguousarray(ops, dtype=np.uint32).reshape(-1, 4)
+        self._off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        st, bad = ctypes.c_int32(), ctypes.c_uint64()
+        self._h = lib().oracle_wo_create(_p32(self._k), _p32(self._v), self._k.size, _p32(self._ops),
+                                         self._ops.shape[0], _p64(self._off), self._off.size - 1,
+                                         ctypes.byref(st), ctypes.byref(bad))
+        if st.value == 1:
+            raise ValueError(f"initial keys not strictly ascending or reserved at {bad.value}")
+        if st.value in (2, 3):
+            raise BatchConflict(int(bad.value))
+
+    def close(self):
+        if self._h:
+            lib().oracle_wo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def answer(self, op_idx, L: int = 0, threads: int = 16):
+        """Returns (results u32[m], pairs u64[m * L] or None) for ops op_idx."""
+        idx = np.ascontiguousarray(op_idx, dtype=np.uint64)
+        res = np.empty(idx.size, dtype=np.uint32)
+        pairs = None
+        if L:
+            pairs = np.full(idx.size * L, np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+        lib().oracle_wo_answer(self._h, _p64(idx), idx.size, _p32(res), _p64(pairs) if L else None, L,
+                               threads)
+        return res, pairs

@@ -104,8 +104,11 @@ class SortedState:
 def batch_expected(initial_keys, initial_values, ops: np.ndarray, offsets: np.ndarray, L: int,
                    num_slots: int):
     """Expected results of the batched form: per batch, every read sees the
-    state before the batch; then the batch's writes apply. Returns
-    (results u32[n], pairs u64[num_slots*L] with empty slots all-ones)."""
+    state before the batch; then the batch's writes apply (a write of a
+    reserved key fails: 0xFFFFFFFF, no effect). A batch whose writes name one
+    key twice raises ValueError: the order of the two writes would decide the
+    result. Returns (results u32[n], pairs u64[num_slots*L] with empty slots
+    all-ones, final SortedState)."""
     st = SortedState(initial_keys, initial_values)
     n = ops.shape[0]
     res = np.zeros(n, dtype=np.uint32)
@@ -125,6 +128,14 @@ def batch_expected(initial_keys, initial_values, ops: np.ndarray, offsets: np.nd
             pairs.reshape(-1, L)[slots] = p.reshape(-1, L)
         ins = ty == 1
         er = ty == 2
+        key = blk[:, 1]
+        bad = (key == 0) | (key == EMPTY)
+        wk = key[(ins | er) & ~bad]
+        if np.unique(wk).size != wk.size:
+            raise ValueError(f"batch {b} writes one key twice")
+        res[idx[(ins | er) & bad]] = EMPTY
+        ins &= ~bad
+        er &= ~bad
         if ins.any():
             res[idx[ins]] = (st.find(blk[ins, 1]) == EMPTY).astype(np.uint32)
         if er.any():
