@@ -73,7 +73,7 @@ def cpu_baseline_config2(w, batch):
     vP, dtP, totalP = cpu_baseline_config2_sharded(w, batch, P)
     if totalP != total1:
         raise RuntimeError(f"sharded CPU baseline disagrees with one map: {totalP} vs {total1} range pairs")
-    return {"value": round(vP, 3), "unit": "M ops/s", "cores": P, "kind": "port",
+    return {"value": round(vP, 3), "unit": "M ops/s", "cores": P, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"1 batch ({ops} ops: {batch.range_lo.size} range-32 + {batch.insert_keys.size} "
                       f"inserts) on the {w.initial_keys.size}-key map: {P} threads over key-range shards "
                       f"{dtP:.2f} s; one thread on one map {ops / dt / 1e6:.3f} M ops/s ({dt:.2f} s)"}
@@ -290,6 +290,24 @@ def run_config2(args):
             "reclaimed_nodes": mem.reclaimed_nodes},
     }
     tree.close()
+    del dbat, out_pairs, out_counts
+    if not args.no_extras:
+        # the metric's other forms, measured in the same run: config 4
+        # (batched form vs plain per-batch launches) and config 5 (served form
+        # vs plain per-request launches, p50/p99 latency) at full size
+        w4 = wl.config4(seed=4, n_initial=n_initial, num_batches=4096)
+        c4 = batched_vs_plain(w4, w4.range_capacity, dev, args.c4_node_width, 1, args.c4_waves)
+        del w4
+        log(f"config4: batched {c4['batched_Mops']} vs plain {c4['plain_per_batch_Mops']} M ops/s")
+        levels = [float(q) for q in args.extras_qps.split(",")]
+        c5 = served_vs_plain(args.served_keys, levels, args.extras_requests, args.step_ops, dev)
+        result["breakdown"]["batched_vs_plain"] = c4
+        result["breakdown"]["served_vs_plain"] = {
+            "keys": c5["keys"], "requests_per_level": c5["requests_per_level"],
+            "levels": [{"qps": lv["qps"],
+                        "served": {k: lv["served"][k] for k in ("Mops", "p50_ms", "p99_ms", "steps")},
+                        "plain": {k: lv["plain"][k] for k in ("Mops", "p50_ms", "p99_ms")}}
+                       for lv in c5["levels"]]}
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline_config2(w, batches[0])
     return result
@@ -360,15 +378,86 @@ def run_config3(args):
             "reclaimed_MB": round(st.reclaimed_nodes * st.node_bytes / 1e6, 1), "timeline": timeline}
 
 
-def run_config4(args):
-    """Batched form (config 4): 4096 batches, sizes log-uniform in [32, 65536],
-    mix 60/20/10/10 find/insert/erase/range (8-128 keys), on a 40M-key tree.
-    The persistent work-queue launch (captured hipGraph, autotuned node width
-    and waves/CU) against the plain per-batch launch path on an identical tree."""
+def batched_vs_plain(w, L, dev, nw: int, mode: int, waves: int):
+    """Config 4 on a fresh tree per path: the whole stream in one persistent
+    launch through the captured hipGraph, against the plain per-batch launch
+    path (C++ loop over the C ABI: snapshot, find, range, insert, erase,
+    release per batch). Both start from the same 40M-key tree."""
     import torch
 
     import mvgpubtree_amd as mv
-    from mvgpubtree_amd import autotune, drivers
+    from mvgpubtree_amd import drivers
+    from mvgpubtree_amd import workloads as wl
+
+    n_ops = sum(b.num_ops for b in w.batches)
+    n0 = w.initial_keys.size
+    pool = int(n0 / 4 * (16 / nw) + 2 * n_ops) + (1 << 20)
+    dk, dv = dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev)
+
+    def fresh_tree():
+        t = mv.GpuVersionedBTree(pool_nodes=pool, node_width=nw)
+        t.insert(dk, dv)
+        t.set_worker_waves(waves)
+        t.set_worker_mode(mode)
+        return t
+
+    # warm both paths (code load, graph build) on a small scratch tree
+    wb = wl.config4(seed=99, n_initial=1_000_000, num_batches=64)
+    scratch = mv.GpuVersionedBTree(pool_nodes=1 << 21, node_width=nw)
+    scratch.insert(dev_tensor(wb.initial_keys, dev), dev_tensor(wb.initial_values, dev))
+    scratch.set_worker_waves(waves)
+    scratch.set_worker_mode(mode)
+    drivers.BatchedRunner(wb.batches, L, dev).run(scratch, use_graph=True)
+    drivers.PerBatchRunner(wb.batches[:8], L, dev).run(scratch)
+    torch.cuda.synchronize()
+    scratch.close()
+
+    br = drivers.BatchedRunner(w.batches, L, dev)
+    tree = fresh_tree()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    br.run(tree, use_graph=True)
+    torch.cuda.synchronize()
+    batched_s = time.perf_counter() - t0
+    flags = tree.error_flags()
+    res_b = br.results.cpu().numpy().view(np.uint32)[: br.nops]
+    ty = br.ops_host[:, 0]
+    failed = int((res_b[(ty == wl.OP_INSERT) | (ty == wl.OP_ERASE)] == 0xFFFFFFFF).sum())
+    tree.close()
+    del br
+
+    pr = drivers.PerBatchRunner(w.batches, L, dev)
+    tree = fresh_tree()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pr.run(tree)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    flags |= tree.error_flags()
+    tree.close()
+    del pr, dk, dv
+    if flags:
+        raise RuntimeError(f"device error flags 0x{flags:x}")
+    return {
+        "batched_Mops": round(n_ops / batched_s / 1e6, 2),
+        "plain_per_batch_Mops": round(n_ops / plain_s / 1e6, 2),
+        "speedup": round(plain_s / batched_s, 2),
+        "batched_ms": round(batched_s * 1e3, 2), "plain_ms": round(plain_s * 1e3, 2),
+        "ops": n_ops, "batches": len(w.batches), "initial_keys": int(n0),
+        "failed_writes": failed,
+        "node_width": nw, "worker_mode": mode, "waves_per_cu": waves,
+    }
+
+
+def run_config4(args):
+    """Batched form (config 4): 4096 batches, sizes log-uniform in [32, 65536],
+    mix 60/20/10/10 find/insert/erase/range (8-128 keys), on a 40M-key tree.
+    The persistent work-queue launch (captured hipGraph, autotuned node width,
+    worker mode and waves/CU) against the plain per-batch launch path on an
+    identical tree."""
+    import torch
+
+    from mvgpubtree_amd import autotune
     from mvgpubtree_amd import workloads as wl
 
     dev = torch.device("cuda:0")
@@ -391,61 +480,10 @@ def run_config4(args):
         best, allr = autotune.TuneResult(widths[0], waves[0], 0.0, 0.0, modes[0]), []
     log(f"autotune best: node width {best.node_width}, worker mode {best.worker_mode}, "
         f"{best.waves_per_cu} waves/CU")
-    n_ops = sum(b.num_ops for b in w.batches)
-    pool = int(args.keys / 4 * (16 / best.node_width) + 2 * n_ops) + (1 << 20)
-    dk, dv = dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev)
-
-    def fresh_tree():
-        t = mv.GpuVersionedBTree(pool_nodes=pool, node_width=best.node_width, retire_capacity=1 << 24)
-        t.insert(dk, dv)
-        t.set_worker_waves(best.waves_per_cu)
-        t.set_worker_mode(best.worker_mode)
-        return t
-
-    # batched form: warm the graph on a scratch run, then one timed launch
-    br = drivers.BatchedRunner(w.batches, L, dev)
-    wb = wl.config4(seed=99, n_initial=1_000_000, num_batches=64)
-    scratch = mv.GpuVersionedBTree(pool_nodes=1 << 21, node_width=best.node_width)
-    scratch.insert(dev_tensor(wb.initial_keys, dev), dev_tensor(wb.initial_values, dev))
-    scratch.set_worker_waves(best.waves_per_cu)
-    scratch.set_worker_mode(best.worker_mode)
-    drivers.BatchedRunner(wb.batches, L, dev).run(scratch, use_graph=True)
-    drivers.PerBatchRunner(wb.batches[:8], L, dev).run(scratch)
-    torch.cuda.synchronize()
-    scratch.close()
-
-    tree = fresh_tree()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    br.run(tree, use_graph=True)
-    torch.cuda.synchronize()
-    batched_s = time.perf_counter() - t0
-    flags = tree.error_flags()
-    res_b = br.results.cpu().numpy().view(np.uint32)
-    tree.close()
-
-    pr = drivers.PerBatchRunner(w.batches, L, dev)
-    tree = fresh_tree()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    pr.run(tree)
-    torch.cuda.synchronize()
-    plain_s = time.perf_counter() - t0
-    flags |= tree.error_flags()
-    tree.close()
-    if flags:
-        raise RuntimeError(f"device error flags 0x{flags:x}")
-    fails = int((res_b == 0xFFFFFFFF)[br.ops_host[:, 0] != wl.OP_FIND].sum())
-    return {
-        "batched_Mops": round(n_ops / batched_s / 1e6, 2),
-        "plain_per_batch_Mops": round(n_ops / plain_s / 1e6, 2),
-        "speedup": round(plain_s / batched_s, 2),
-        "batched_ms": round(batched_s * 1e3, 2), "plain_ms": round(plain_s * 1e3, 2),
-        "ops": n_ops, "batches": len(w.batches), "failed_writes": fails,
-        "tuned": {"node_width": best.node_width, "worker_mode": best.worker_mode, "waves_per_cu": best.waves_per_cu},
-        "tuning": [{"nw": r.node_width, "mode": r.worker_mode, "waves": r.waves_per_cu, "Mops": round(r.mops, 1)}
-                   for r in allr],
-    }
+    out = batched_vs_plain(w, L, dev, best.node_width, best.worker_mode, best.waves_per_cu)
+    out["tuning"] = [{"nw": r.node_width, "mode": r.worker_mode, "waves": r.waves_per_cu, "Mops": round(r.mops, 1)}
+                     for r in allr]
+    return out
 
 
 def gpu_unique_keys(n: int, extra: int, seed: int, dev):
@@ -468,29 +506,26 @@ def gpu_unique_keys(n: int, extra: int, seed: int, dev):
     return u32[:n].contiguous(), u32[n:].contiguous()
 
 
-def run_config5(args):
-    """Served form (config 5): 500M-key tree resident in HBM; open-loop Poisson
-    arrivals at fixed QPS steps; request sizes log-uniform in [1, 16384] ops,
+def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev):
+    """Served form (config 5): n-key tree resident in HBM; open-loop Poisson
+    arrivals at each QPS level; request sizes log-uniform in [1, 16384] ops,
     80% find / 10% range (~32 keys) / 10% insert, one snapshot per request;
-    continuous batching vs the plain per-request launch path."""
+    continuous batching against the plain per-request launch path (C++ loop:
+    each request issued at its arrival or when the previous one completes,
+    and waited for) on the same requests."""
     import torch
 
     import mvgpubtree_amd as mv
     from mvgpubtree_amd import serving
     from mvgpubtree_amd import workloads as wl
 
-    dev = torch.device("cuda:0")
-    n = args.served_keys
-    levels = [float(q) for q in args.qps.split(",")]
-    n_req = args.requests
     t0 = time.perf_counter()
     init, fresh = gpu_unique_keys(n, (len(levels) + 1) * n_req * 2000, seed=5, dev=dev)
     vals = init ^ torch.tensor(wl.VALUE_XOR - 2**32, dtype=torch.int32, device=dev)
     pool = int(n / 6) + (8 << 20)
-    tree = mv.GpuVersionedBTree(pool_nodes=pool, retire_capacity=1 << 24)
-    # Requests are small (mean about 1.7K ops), so the chunked worker would take
-    # runs of one op anyway; the one-op worker has the smaller register
-    # footprint and measured faster for the served steps.
+    tree = mv.GpuVersionedBTree(pool_nodes=pool)
+    # Requests are small (mean about 1.7K ops), so runs would be one op anyway:
+    # the one-op worker has the smaller register footprint.
     tree.set_worker_mode(0)
     torch.cuda.synchronize()
     tb = time.perf_counter()
@@ -516,18 +551,16 @@ def run_config5(args):
     serving.ContinuousBatchingServer(tree, w, dev, stream=stream).run()
     w = requests(2e3, 16)
     serving.ContinuousBatchingServer(tree, w, dev, stream=stream).run()
-    srv = serving.PerRequestServer(tree, w, dev)
-    for i in range(16):
-        srv.runner.run_one(tree, i)
+    serving.PerRequestServer(tree, w, dev).run()
     torch.cuda.synchronize()
     tree.reclaim()
     out = []
     for q in levels:
         reqs = requests(q, n_req)
-        s_sum, _ = serving.ContinuousBatchingServer(tree, reqs, dev, max_step_ops=args.step_ops,
-                                                    stream=stream).run()
-        p_sum, _ = serving.PerRequestServer(tree, reqs.__class__(
-            requests(q, n_req).batches, reqs.arrivals, reqs.range_capacity), dev).run()
+        s_sum, _ = serving.ContinuousBatchingServer(tree, reqs, dev, max_step_ops=step_ops, stream=stream).run()
+        # the same requests (sizes, mix, arrivals, keys): its inserts then
+        # replace values the served run inserted, the same work per key
+        p_sum, _ = serving.PerRequestServer(tree, reqs, dev).run()
         tree.reclaim()
         out.append({"qps": q, "served": s_sum, "plain": p_sum})
         log(f"qps {q:.0f}: served {s_sum['Mops']} M ops/s p50 {s_sum['p50_ms']} ms p99 {s_sum['p99_ms']} ms"
@@ -543,6 +576,23 @@ def run_config5(args):
     return {"keys": n, "build_Minserts_per_s": round(n / build_s / 1e6, 1), "requests_per_level": n_req,
             "levels": out, "pool_live_GB": round(mem.live_bytes / 1e9, 2),
             "pool_peak_GB": round(mem.peak_bytes / 1e9, 2)}
+
+
+def run_config5(args):
+    import torch
+    levels = [float(q) for q in args.qps.split(",")]
+    return served_vs_plain(args.served_keys, levels, args.requests, args.step_ops, torch.device("cuda:0"))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def load_traffic(kernel: str, node_width: int):
@@ -597,6 +647,12 @@ def main():
     ap.add_argument("--modes", default="0,1", help="config 4: worker modes to tune (0 one op per tile, 1 runs)")
     ap.add_argument("--widths", default="16,32,64", help="config 4: node widths to tune")
     ap.add_argument("--waves", default="8,16,24,32", help="config 4: waves/CU to tune")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default run: skip the config-4 and config-5 legs of the breakdown")
+    ap.add_argument("--extras-qps", default="8000,32000", help="default run: config-5 QPS levels")
+    ap.add_argument("--extras-requests", type=int, default=1000, help="default run: config-5 requests per level")
+    ap.add_argument("--c4-node-width", type=int, default=32, help="default run: config-4 node width")
+    ap.add_argument("--c4-waves", type=int, default=8, help="default run: config-4 worker waves per CU")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()

@@ -53,17 +53,28 @@ typedef enum mvbt_kind {
 #define MVBT_FLAG_CORRUPT 0x20u        /* a traversal left the pool or hit its step bound */
 #define MVBT_FLAG_SNAPSHOT_ID 0x40u    /* a snapshot id returned without waiting differs from the
                                           device clock (the library's own consistency check) */
+#define MVBT_FLAG_STAMP_ORDER 0x80u    /* a write stamped by its launch (batched form, mixed calls)
+                                          met a leaf stamped later by an overlapping writer (user
+                                          kernels through a device view); it was applied at the
+                                          later stamp */
+#define MVBT_FLAG_BAD_BATCH 0x100u     /* mvbt_run_batches rejected a call (a batch writes one key
+                                          twice, an unknown op type, or offsets that disagree
+                                          with num_ops); nothing of it was applied */
 
 /* Operation flags. */
 #define MVBT_CONCURRENT 0x1u /* readers may run while writers run on another stream
-                                (reference `bool concurrent`): use coherent loads and
-                                validate leaf versions */
+                                (reference `bool concurrent`): coherent loads; a snapshot
+                                read validates each leaf against its version word and lock
+                                word, a read of the latest version (MVBT_LATEST, and every
+                                read of a plain tree) reads each leaf under its lock */
 
 typedef struct mvbt_config {
   uint32_t kind;            /* mvbt_kind */
   uint32_t node_width;      /* pairs per node: 16 (128 B), 32 or 64; 0 = 16 */
   uint64_t pool_nodes;      /* node slots (< 2^29); 0 = 1<<22 */
-  uint64_t retire_capacity; /* retire ring entries (power of two); 0 = 1<<22 */
+  uint64_t retire_capacity; /* retire ring entries (power of two); raised to at least
+                               pool_nodes (every copy awaiting reclamation holds a pool slot,
+                               so the ring then never fills); 0 = that minimum */
   int32_t device;           /* HIP device ordinal; -1 = current device */
   uint32_t reserved[3];
 } mvbt_config;
@@ -193,8 +204,9 @@ mvbt_status mvbt_bulk_build(mvbt_tree* tree, const uint32_t* keys, const uint32_
  * deterministic (the reference snapshots inside the kernel). Plain trees: the
  * queries see the latest state (any linearization, as in the reference). Range
  * i writes at most L pairs to result[i*L ...] (result may be NULL) and its true
- * count to counts[i] (counts may be NULL). Calls on one handle must not
- * overlap on different streams. */
+ * count to counts[i] (counts may be NULL). The launch runs alone among this
+ * handle's writers: it waits for writes issued earlier on other streams, and
+ * writes issued later on other streams wait for it. */
 mvbt_status mvbt_concurrent_insert_range(mvbt_tree* tree, const uint32_t* keys, const uint32_t* values,
                                          uint64_t num_insertions, const uint32_t* lower, const uint32_t* upper,
                                          uint64_t num_ranges, mvbt_pair* result, uint32_t* counts, uint32_t L,
@@ -241,15 +253,24 @@ typedef struct mvbt_op {
  * btree_kernels.hpp:31-306) to any mix, deterministically. Runs num_batches batches in
  * one persistent work-queue launch. batch_offsets (device, num_batches + 1 entries) are
  * absolute indices into ops: batch b is ops[offsets[b] .. offsets[b+1]), results go to
- * the same indices, and offsets[num_batches] - offsets[0] must equal num_ops (so a
- * launch can cover any slice of a larger pre-staged op array). Batch b reads (find, range) at snapshot base + b, i.e.
- * it sees exactly the writes of the batches before it; identical to calling
+ * the same indices (so `results` must cover index offsets[num_batches] - 1), and
+ * offsets[num_batches] - offsets[0] must equal num_ops (so a launch can cover any slice
+ * of a larger pre-staged op array). Batch b reads (find, range) at snapshot base + b,
+ * i.e. it sees exactly the writes of the batches before it; identical to calling
  * take_snapshot + find/range + insert/erase batch by batch. results[i]: find -> value
  * or MVBT_NOT_FOUND; range -> true count (at most L pairs written); insert -> 1 new
- * key, 0 value replaced; erase -> 1 erased, 0 absent; failed write -> 0xFFFFFFFF.
- * The tree clock advances by num_batches; *snapshot_base (device pointer, may be
- * NULL) receives base. Versioned trees only; scratch for num_batches is allocated
- * on first use (synchronous then). */
+ * key, 0 value replaced; erase -> 1 erased, 0 absent; write of a reserved key ->
+ * 0xFFFFFFFF (skipped, MVBT_FLAG_INVALID_KEY).
+ * The writes of one batch must name distinct keys (their order inside the batch is
+ * not defined); a call in which a batch writes one key twice, an op has an unknown
+ * type, or the offsets disagree with num_ops is rejected on the device: nothing is
+ * applied, results are left unchanged and MVBT_FLAG_BAD_BATCH is raised.
+ * The tree clock advances by num_batches (also for a rejected call); *snapshot_base
+ * (device pointer, may be NULL) receives base. The launch runs alone among this
+ * handle's writers: it waits for writes issued earlier on other streams, and writes
+ * issued later on other streams (other run_batches calls included) wait for it.
+ * Versioned trees only; scratch for num_batches and num_ops is allocated on first use
+ * and when a call needs more (synchronous then). */
 mvbt_status mvbt_run_batches(mvbt_tree* tree, const mvbt_op* ops, uint64_t num_ops,
                              const uint64_t* batch_offsets, uint32_t num_batches,
                              uint32_t* results, mvbt_pair* pairs, uint64_t pair_slots, uint32_t L,

@@ -21,7 +21,7 @@ from dataclasses import dataclass
 __all__ = [
     "GpuBlinkTree", "GpuVersionedBTree", "MvbtError", "LATEST", "NOT_FOUND",
     "FLAG_POOL_EXHAUSTED", "FLAG_RETIRE_OVERFLOW", "FLAG_INVALID_KEY", "FLAG_VERSION_GAP",
-    "FLAG_RANGE_OVERFLOW", "FLAG_CORRUPT", "FLAG_SNAPSHOT_ID",
+    "FLAG_RANGE_OVERFLOW", "FLAG_CORRUPT", "FLAG_SNAPSHOT_ID", "FLAG_STAMP_ORDER", "FLAG_BAD_BATCH",
     "CONCURRENT", "library_path", "load_library", "EXPORTED_SYMBOLS",
 ]
 
@@ -40,6 +40,8 @@ FLAG_VERSION_GAP = 0x8
 FLAG_RANGE_OVERFLOW = 0x10
 FLAG_CORRUPT = 0x20
 FLAG_SNAPSHOT_ID = 0x40
+FLAG_STAMP_ORDER = 0x80
+FLAG_BAD_BATCH = 0x100
 PLAIN, VERSIONED = 0, 1
 
 _STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "UNSUPPORTED", 5: "DEVICE_FLAGS"}
@@ -398,12 +400,14 @@ class GpuVersionedBTree(_TreeBase):
         """Batched form: `ops` is an int32 device tensor of shape (n, 4) holding
         {type, key, arg, slot} (OP_* types); `batch_offsets` an int64 device
         tensor of num_batches + 1 absolute op indices into `ops`; results go to
-        the same indices. `num_ops` = offsets[-1] - offsets[0] (default: all of
-        `ops`). Returns the results tensor."""
+        the same indices (the default results tensor has one entry per row of
+        `ops`). `num_ops` = offsets[-1] - offsets[0] (default: all of `ops`).
+        A batch must not write one key twice (the call is then rejected:
+        FLAG_BAD_BATCH). Returns the results tensor."""
         torch = _torch()
         n = ops.shape[0] if num_ops is None else num_ops
         if results is None:
-            results = torch.empty(n, dtype=torch.int32, device=ops.device)
+            results = torch.empty(ops.shape[0], dtype=torch.int32, device=ops.device)
         nb = batch_offsets.numel() - 1
         _check(load_library().mvbt_run_batches(
             self._h, _ptr(ops, 4, "ops"), n, _ptr(batch_offsets, 8, "batch_offsets"), nb,

@@ -4,6 +4,8 @@
   * ``mvgpubtree_amd/libmvbt.so``   - kernels + C ABI (include/mvbt.h)
   * ``mvgpubtree_amd/capi_example`` - C++ program using include/gpu_btree.h
   * ``mvgpubtree_amd/device_api_example`` - user kernels on include/mvbt_device.hpp
+  * ``mvgpubtree_amd/libmvbt_driver.so`` - the plain per-batch / per-request
+    caller loops over the C ABI (baselines of the batched and served forms)
 """
 from __future__ import annotations
 
@@ -69,10 +71,25 @@ def build_device_example(verbose: bool = False) -> str:
     return out
 
 
+def build_driver(verbose: bool = False) -> str:
+    """The plain per-batch / per-request caller loops over the C ABI
+    (libmvbt_driver.so, host code), the baselines of the batched and served forms."""
+    out = os.path.join(PKG, "libmvbt_driver.so")
+    tmp = out + ".tmp"
+    src = os.path.join(CSRC, "per_batch_driver.cpp")
+    cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-Xclang", "-target-feature",
+           "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument", "-o", tmp, src, f"-L{PKG}", "-lmvbt",
+           f"-Wl,-rpath,{PKG}"]
+    _run(cmd, verbose)
+    os.replace(tmp, out)
+    return out
+
+
 def build(verbose: bool = False) -> None:
     build_lib(verbose)
     build_example(verbose)
     build_device_example(verbose)
+    build_driver(verbose)
 
 
 if __name__ == "__main__":

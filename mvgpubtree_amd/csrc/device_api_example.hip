@@ -1,9 +1,12 @@
-// User kernels on the device-side cooperative API (include/mvbt_device.hpp):
-// cooperative_insert from a user kernel, take_snapshot(tile) inside a kernel,
-// then one kernel where some tiles insert while other tiles read that snapshot
-// with cooperative_find and concurrent_cooperative_range_query. The expected
-// values follow from closed forms (keys 2(i+1), value = key, as in the
-// reference unit tests test/unittest_versioning.cu:82-95). Prints "OK".
+// User kernels on the device-side cooperative API (include/mvbt_device.hpp),
+// written with the reference's argument lists: the three insert forms from a
+// user kernel, take_snapshot(tile) inside a kernel, then a kernel where some
+// tiles insert while others read that snapshot (cooperative_find with a
+// timestamp, concurrent_cooperative_range_query), then both erase forms beside
+// concurrent_cooperative_find at the snapshot and latest-version finds; and
+// the plain tree's insert, concurrent erase, find and range query. The
+// expected values follow from closed forms (keys 2(i+1), value = key, as in
+// the reference unit tests test/unittest_versioning.cu:82-95). Prints "OK".
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -31,17 +34,25 @@
 
 constexpr int B = 16;
 using tree_t = GpuBTree::device::versioned_tree<B>;
+using plain_t = GpuBTree::device::blink_tree<B>;
 using tile_t = GpuBTree::device::tile<B>;
 
 __device__ uint32_t tile_id() { return (blockIdx.x * blockDim.x + threadIdx.x) / B; }
 __device__ uint32_t tile_count() { return gridDim.x * blockDim.x / B; }
 
+// The reference's insert forms (gpu_versioned_blink_tree.hpp:306-330), one per
+// key in turn.
 __global__ void insert_kernel(mvbt_device_handle h, const uint32_t* keys, uint32_t n) {
   tree_t tree(h);
   tile_t t;
-  auto ctx = tree.begin_writes(t);
-  for (uint32_t i = tile_id(); i < n; i += tile_count()) tree.cooperative_insert(keys[i], keys[i], t, ctx);
-  tree.end_writes(ctx, t);
+  auto allocator = tree.begin_writes(t);
+  auto reclaimer = tree.reclaimer(allocator);
+  for (uint32_t i = tile_id(); i < n; i += tile_count()) {
+    if (i % 3 == 0) tree.cooperative_insert(keys[i], keys[i], t, allocator, reclaimer);
+    else if (i % 3 == 1) tree.cooperative_insert_in_place(keys[i], keys[i], t, allocator);
+    else tree.cooperative_insert_out_of_place(keys[i], keys[i], t, allocator, reclaimer);
+  }
+  tree.end_writes(allocator, t);
 }
 
 __global__ void snapshot_kernel(mvbt_device_handle h, uint32_t* out) {
@@ -54,30 +65,98 @@ __global__ void snapshot_kernel(mvbt_device_handle h, uint32_t* out) {
 }
 
 // Ops [0, n): find keys[i] at snapshot; [n, 2n): range [keys[i], keys[i]+2L) at
-// snapshot into pairs + i*L; [2n, 2n + late): insert keys[half + j].
+// snapshot into pairs + i*L (the reference's unbounded form: such a range
+// holds at most L keys); [2n, 3n - half): insert keys[half + j].
 __global__ void mixed_kernel(mvbt_device_handle h, const uint32_t* keys, uint32_t n, uint32_t half,
                              const uint32_t* snap, uint32_t* found, mvbt_pair* pairs,
                              uint32_t* counts, uint32_t L) {
   tree_t tree(h);
   tile_t t;
   const uint32_t s = *snap;
-  auto ctx = tree.begin_writes(t);
+  auto allocator = tree.begin_writes(t);
+  auto reclaimer = tree.reclaimer(allocator);
   const uint32_t total = 3 * n;
   for (uint32_t i = tile_id(); i < total; i += tile_count()) {
     if (i < n) {
-      uint32_t v = tree.cooperative_find(keys[i], t, s, /*concurrent=*/true);
+      uint32_t v = tree.cooperative_find(keys[i], t, allocator, s, /*concurrent=*/true);
       if (t.leader()) found[i] = v;
     } else if (i < 2 * n) {
       uint32_t q = i - n;
-      uint32_t c = tree.concurrent_cooperative_range_query(keys[q], keys[q] + 2 * L, t, s,
-                                                           pairs + (size_t)q * L, L, true);
+      uint32_t c = tree.concurrent_cooperative_range_query(keys[q], keys[q] + 2 * L, t, allocator, s,
+                                                           pairs + (size_t)q * L);
       if (t.leader()) counts[q] = c;
     } else if (i - 2 * n < n - half) {
       uint32_t k = keys[half + (i - 2 * n)];
-      tree.cooperative_insert(k, k, t, ctx);
+      tree.cooperative_insert(k, k, t, allocator, reclaimer);
     }
   }
-  tree.end_writes(ctx, t);
+  tree.end_writes(allocator, t);
+}
+
+// After the mixed kernel: keys[i] for i % 4 == 0 are erased (both erase
+// forms) while other tiles read: concurrent_cooperative_find at the
+// snapshot, and the latest version with cooperative_find(.., concurrent).
+// Latest reads of keys that are never erased must hit; snapshot reads must
+// equal the snapshot. erased[i] gets the erase result.
+__global__ void erase_kernel(mvbt_device_handle h, const uint32_t* keys, uint32_t n, const uint32_t* snap,
+                             uint32_t* at_snap, uint32_t* latest, uint32_t* erased) {
+  tree_t tree(h);
+  tile_t t;
+  const uint32_t s = *snap;
+  auto allocator = tree.begin_writes(t);
+  auto reclaimer = tree.reclaimer(allocator);
+  for (uint32_t i = tile_id(); i < 2 * n; i += tile_count()) {
+    if (i < n) {
+      const uint32_t k = keys[i];
+      if (i % 4 == 0) {
+        const bool ok = (i % 8 == 0) ? tree.cooperative_erase(k, t, allocator, reclaimer)
+                                     : tree.cooperative_erase(k, t, allocator, /*concurrent=*/true);
+        if (t.leader()) erased[i] = ok ? 1u : 0u;
+      } else {
+        const uint32_t v = tree.cooperative_find(k, t, allocator, /*concurrent=*/true);
+        if (t.leader()) latest[i] = v;
+      }
+    } else {
+      const uint32_t q = i - n;
+      const uint32_t v = tree.concurrent_cooperative_find(keys[q], t, allocator, s);
+      if (t.leader()) at_snap[q] = v;
+    }
+  }
+  tree.end_writes(allocator, t);
+}
+
+// Plain tree (gpu_blink_tree.hpp:269-390): inserts, then erases of every
+// fourth key beside concurrent finds and range queries of the others.
+__global__ void plain_insert_kernel(mvbt_device_handle h, const uint32_t* keys, uint32_t n) {
+  plain_t tree(h);
+  tile_t t;
+  auto allocator = tree.begin_writes(t);
+  for (uint32_t i = tile_id(); i < n; i += tile_count()) tree.cooperative_insert(keys[i], keys[i], t, allocator);
+  tree.end_writes(allocator, t);
+}
+__global__ void plain_mixed_kernel(mvbt_device_handle h, const uint32_t* keys, uint32_t n, uint32_t* found,
+                                   uint32_t* erased, mvbt_pair* pairs, uint32_t* counts) {
+  plain_t tree(h);
+  tile_t t;
+  auto allocator = tree.begin_writes(t);
+  for (uint32_t i = tile_id(); i < 2 * n; i += tile_count()) {
+    if (i < n) {
+      const uint32_t k = keys[i];
+      if (i % 4 == 0) {
+        const bool ok = tree.cooperative_erase(k, t, allocator, /*concurrent=*/true);
+        if (t.leader()) erased[i] = ok ? 1u : 0u;
+      } else {
+        const uint32_t v = tree.cooperative_find(k, t, allocator, /*concurrent=*/true);
+        if (t.leader()) found[i] = v;
+      }
+    } else {
+      // [k + 1, k + 2): holds no key (keys are even), so the count is 0
+      const uint32_t q = i - n;
+      const uint32_t c = tree.cooperative_range_query(keys[q] + 1, keys[q] + 2, t, allocator, pairs + q, true);
+      if (t.leader()) counts[q] = c;
+    }
+  }
+  tree.end_writes(allocator, t);
 }
 
 int main(int argc, char** argv) {
@@ -157,6 +236,69 @@ int main(int argc, char** argv) {
   MVBT_OK_(mvbt_error_flags(tree, &flags));
   if (nk != n) fail("key count", 0, (uint32_t)nk, n);
   if (flags) fail("device flags", 0, flags, 0);
+
+  // erases beside snapshot and latest readers
+  uint32_t *d_at_snap, *d_latest, *d_erased;
+  HIP_OK(hipMalloc(&d_at_snap, n * 4));
+  HIP_OK(hipMalloc(&d_latest, n * 4));
+  HIP_OK(hipMalloc(&d_erased, n * 4));
+  hipLaunchKernelGGL(erase_kernel, dim3(1024), dim3(256), 0, 0, h, d_keys, n, d_snap, d_at_snap, d_latest, d_erased);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipDeviceSynchronize());
+  std::vector<uint32_t> at_snap(n), lat(n), erased(n);
+  HIP_OK(hipMemcpy(at_snap.data(), d_at_snap, n * 4, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(lat.data(), d_latest, n * 4, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(erased.data(), d_erased, n * 4, hipMemcpyDeviceToHost));
+  MVBT_OK_(mvbt_find(tree, d_keys, d_found, n, MVBT_LATEST, nullptr, 0));
+  HIP_OK(hipMemcpy(latest.data(), d_found, n * 4, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t k = keys[i];
+    const uint32_t want_snap = k <= max_key ? k : MVBT_NOT_FOUND;
+    if (at_snap[i] != want_snap) fail("concurrent find@snapshot", i, at_snap[i], want_snap);
+    if (i % 4 == 0) {
+      if (erased[i] != 1) fail("erase result", i, erased[i], 1);
+      if (latest[i] != MVBT_NOT_FOUND) fail("erased key still found", i, latest[i], MVBT_NOT_FOUND);
+    } else {
+      if (lat[i] != k) fail("concurrent find@latest", i, lat[i], k);
+      if (latest[i] != k) fail("find@latest after erases", i, latest[i], k);
+    }
+  }
+  MVBT_OK_(mvbt_validate(tree, &nk, nullptr));
+  const uint32_t n_erased = (n + 3) / 4;
+  if (nk != n - n_erased) fail("key count after erases", 0, (uint32_t)nk, n - n_erased);
+  MVBT_OK_(mvbt_error_flags(tree, &flags));
+  if (flags) fail("device flags", 1, flags, 0);
+  MVBT_OK_(mvbt_destroy(tree));
+
+  // plain tree
+  cfg.kind = MVBT_PLAIN;
+  MVBT_OK_(mvbt_create(&cfg, &tree));
+  MVBT_OK_(mvbt_device_view(tree, &h));
+  hipLaunchKernelGGL(plain_insert_kernel, dim3(512), dim3(256), 0, 0, h, d_keys, n);
+  hipLaunchKernelGGL(plain_mixed_kernel, dim3(1024), dim3(256), 0, 0, h, d_keys, n, d_found, d_erased, d_pairs,
+                     d_counts);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipDeviceSynchronize());
+  HIP_OK(hipMemcpy(lat.data(), d_found, n * 4, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(erased.data(), d_erased, n * 4, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(counts.data(), d_counts, n * 4, hipMemcpyDeviceToHost));
+  MVBT_OK_(mvbt_find(tree, d_keys, d_found, n, MVBT_LATEST, nullptr, 0));
+  HIP_OK(hipMemcpy(latest.data(), d_found, n * 4, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t k = keys[i];
+    if (counts[i] != 0) fail("plain range count", i, counts[i], 0);
+    if (i % 4 == 0) {
+      if (erased[i] != 1) fail("plain erase result", i, erased[i], 1);
+      if (latest[i] != MVBT_NOT_FOUND) fail("plain erased key found", i, latest[i], MVBT_NOT_FOUND);
+    } else {
+      if (lat[i] != k) fail("plain concurrent find", i, lat[i], k);
+      if (latest[i] != k) fail("plain find after erases", i, latest[i], k);
+    }
+  }
+  MVBT_OK_(mvbt_validate(tree, &nk, nullptr));
+  if (nk != n - n_erased) fail("plain key count", 0, (uint32_t)nk, n - n_erased);
+  MVBT_OK_(mvbt_error_flags(tree, &flags));
+  if (flags) fail("plain device flags", 0, flags, 0);
   MVBT_OK_(mvbt_destroy(tree));
   if (errors) {
     std::printf("FAILED %d checks\n", errors);

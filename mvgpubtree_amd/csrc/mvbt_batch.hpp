@@ -16,6 +16,13 @@
 // A tile that must wait polls F once per pass of its outer loop and never
 // spins inside a branch (the tiles of a wave run in lockstep).
 //
+// A batch's writes must name distinct keys: two writes of one key inside a
+// batch would make the result depend on which tile runs first. The count
+// pass checks this with a hash table over (batch, key) of the call's write
+// operations and rejects the whole call (kErrBadBatch; nothing is applied,
+// results are left as they were) when a key repeats, an operation type is
+// unknown, or the offsets disagree with the operation count.
+//
 // Reference counterpart: the ratio-partitioned mixed kernels
 // concurrent_insert_range_kernel / concurrent_find_erase_kernel_versioned
 // (btree_kernels.hpp:31-306), which take per-tile snapshots inside the
@@ -45,12 +52,16 @@ struct alignas(128) batch_job {
   unsigned long long pair_slots;  // capacity of `pairs` in slots of L pairs
   uint32_t* base_out;    // optional: receives clock0
   uint32_t clock0;
-  uint32_t pad0;
-  // library-owned scratch, capacity batches each
+  uint32_t hash_mask;    // conflict table size - 1 (a power of two >= 2 * ops)
+  unsigned long long first_op;  // offsets[0]
+  // library-owned scratch: capacity batches each, and the conflict table
   uint32_t* nwrites;
   uint32_t* done;
   int32_t* advance;
-  unsigned long long pad1[3];
+  uint32_t* hash;        // hash_mask + 1 entries: 0 empty, else 1 + op index - first_op
+  uint32_t* op_batch;    // batch of op first_op + i, written by the count pass
+  uint32_t bad;          // nonzero: the call was rejected and runs nothing
+  uint32_t pad1[1];
   // hot runtime words, one line each
   unsigned long long next_op;  // next unclaimed operation index
   unsigned long long pad2[15];
@@ -71,6 +82,7 @@ struct batch_params {
   uint64_t* pairs;
   unsigned long long pair_slots;
   uint32_t* base_out;
+  uint32_t hash_mask;
 };
 
 // 1. Parameters in, clock range reserved, queue reset.
@@ -85,24 +97,80 @@ __global__ void batch_setup_kernel(batch_job* job, tree_state* st, batch_params 
   job->pair_slots = p.pair_slots;
   job->base_out = p.base_out;
   job->clock0 = __hip_atomic_fetch_add(&st->clock, p.nb, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  job->hash_mask = p.hash_mask;
   // Operation indices are absolute: batch b covers ops [offsets[b], offsets[b+1]),
-  // results go to the same indices. The caller checked offsets[nb] - offsets[0]
-  // equals the op count it passed.
-  job->next_op = p.offsets[0];
-  job->nops = p.offsets[p.nb];
+  // results go to the same indices; offsets[nb] - offsets[0] must equal the op
+  // count the caller passed (the size its buffers were checked against).
+  const unsigned long long first = p.offsets[0], last = p.offsets[p.nb];
+  const bool ok = last >= first && last - first == p.nops;
+  job->first_op = first;
+  job->next_op = first;
+  job->nops = ok ? last : first;
+  job->bad = ok ? 0u : 1u;
+  if (!ok) atomic_or32(&st->error_flags, kErrBadBatch);
   for (uint32_t c = 0; c < kFrontierCopies; ++c) job->frontier[c * 32] = -1;
 }
 
-// 2. Writes per batch (one block per batch, grid-stride over batches).
-__global__ void batch_count_kernel(batch_job* job) {
+// 1'. Clears the conflict table (a fill kernel, so the launch graph holds no
+// memset node).
+__global__ void batch_clear_kernel(batch_job* job) {
+  const uint32_t n = job->hash_mask + 1;
+  uint32_t* h = job->hash;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) h[i] = 0;
+}
+
+MVBT_DEV uint32_t batch_of(const unsigned long long* offsets, uint32_t nb, unsigned long long i);
+
+MVBT_DEV uint32_t conflict_hash(uint32_t key, uint32_t b) {
+  uint32_t h = key * 0x9E3779B1u ^ (b * 0x7FEB352Du + 0x165667B1u);
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return h;
+}
+
+// Records write op i (batch b, valid key) in the conflict table; false when
+// batch b already writes `key`. Linear probing; the table has at least twice
+// as many entries as the call has operations, so a free entry always exists.
+MVBT_DEV bool claim_write(const batch_job* job, unsigned long long i, uint32_t b, uint32_t key) {
+  const uint32_t mask = job->hash_mask;
+  uint32_t* table = job->hash;
+  const uint32_t me = (uint32_t)(i - job->first_op) + 1u;
+  uint32_t h = conflict_hash(key, b) & mask;
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    uint32_t cur = ld32_coherent(table + h);
+    if (cur == 0) {
+      cur = atomic_cas32(table + h, 0u, me);
+      if (cur == 0) return true;
+    }
+    const unsigned long long j = job->first_op + cur - 1u;
+    if (job->ops[j].key == key && batch_of(job->offsets, job->nb, j) == b) return false;
+    h = (h + 1) & mask;
+  }
+  return true;
+}
+
+// 2. Writes per batch (one block per batch, grid-stride over batches), and
+// the call's checks: known op types, ordered offsets, distinct write keys
+// per batch.
+__global__ void batch_count_kernel(batch_job* job, tree_state* st) {
   __shared__ uint32_t partial[4];
   const uint32_t nb = job->nb;
+  if (ld32_coherent(&job->bad)) return;  // offsets disagree with the op count
   for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
     const unsigned long long lo = job->offsets[b], hi = job->offsets[b + 1];
     uint32_t c = 0;
+    bool bad = hi < lo;
     for (unsigned long long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      uint32_t ty = job->ops[i].type;
-      c += (ty == kOpInsert || ty == kOpErase) ? 1u : 0u;
+      const op_record o = job->ops[i];
+      const bool w = o.type == kOpInsert || o.type == kOpErase;
+      c += w ? 1u : 0u;
+      job->op_batch[i - job->first_op] = b;
+      if (o.type > kOpRange || (w && valid_key(o.key) && !claim_write(job, i, b, o.key))) bad = true;
+    }
+    if (bad) {
+      atomic_or32(&job->bad, 1u);
+      atomic_or32(&st->error_flags, kErrBadBatch);
     }
     // wave reduction via ballot-free shuffles
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
@@ -139,27 +207,7 @@ __global__ void batch_finish_kernel(batch_job* job) {
   if (threadIdx.x == 0 && blockIdx.x == 0 && job->base_out) *job->base_out = job->clock0;
 }
 
-// Batch of op i for a caller whose previous op was in batch `from` (op
-// indices only grow for a tile): gallop forward from `from`, then bisect, so
-// the usual case (same or next batch) costs one or two loads instead of
-// log2(nb).
-MVBT_DEV uint32_t batch_after(const unsigned long long* offsets, uint32_t nb, unsigned long long i, uint32_t from) {
-  uint32_t lo = from, step = 1;  // offsets[lo] <= i
-  uint32_t hi = lo + step;
-  while (hi < nb && offsets[hi] <= i) {
-    lo = hi;
-    step <<= 1;
-    hi = lo + step;
-  }
-  if (hi > nb) hi = nb;  // offsets[lo] <= i < offsets[hi]
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (offsets[mid] <= i) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
+// Batch of op i (offsets[0] <= i < offsets[nb]), by bisection.
 MVBT_DEV uint32_t batch_of(const unsigned long long* offsets, uint32_t nb, unsigned long long i) {
   uint32_t lo = 0, hi = nb;  // offsets[lo] <= i < offsets[hi] (i in [offsets[0], offsets[nb]))
   while (hi - lo > 1) {
@@ -181,6 +229,7 @@ MVBT_DEV uint32_t batch_of(const unsigned long long* offsets, uint32_t nb, unsig
 // queue whose waves are not resident could block every resident tile.
 template <int NW>
 __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_job* job) {
+  if (ld32_coherent(&job->bad)) return;  // rejected call: nothing runs
   const tree_view<NW, true> tv(tp);
   using node_t = node<NW, true>;
   tile<NW> t;
@@ -196,6 +245,7 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
   int32_t* const my_frontier = job->frontier + (wave % kFrontierCopies) * 32;
   uint32_t backoff = 1;  // s_sleep units while the whole wave waits
   write_ctx ctx = tv.begin_writes(t);
+  ctx.track = 0;          // stamps are fixed per batch: clock0 + b + 1
   int32_t seen = -1;      // last frontier value observed (tile-uniform)
   bool have = false;      // tile holds an operation (tile-uniform)
   unsigned long long idx = 0;
@@ -218,7 +268,7 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
         idx = base + (unsigned long long)__builtin_popcountll(below);
         if (idx >= nops) break;  // queue drained
         op = ops[idx];
-        b = batch_after(job->offsets, nb, idx, b);
+        b = job->op_batch[idx - job->first_op];
         have = true;
       }
     }
@@ -259,11 +309,11 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
     if (route) leaf = tv.template descend<true, true>(rkey, n, t, &par);
     uint32_t r;
     if (op.type == kOpFind) {
-      r = route ? tv.template find_in<true>(op.key, clock0 + b, n, leaf, t) : kEmpty;
+      r = route ? tv.template find_in<true, false>(op.key, clock0 + b, n, leaf, t) : kEmpty;
     } else if (op.type == kOpRange) {
       uint64_t* out = (pairs && op.slot < pair_slots) ? pairs + (size_t)op.slot * L : nullptr;
       if (pairs && op.slot >= pair_slots) tv.flag(kErrCorrupt, t);
-      r = route ? tv.template range_in<true>(rkey, op.arg, clock0 + b, n, leaf, out, L, t) : 0u;
+      r = route ? tv.template range_in<true, false>(rkey, op.arg, clock0 + b, n, leaf, out, L, t) : 0u;
       if (r > L) tv.flag(kErrRangeOverflow, t);
     } else {
       ctx.cur = clock0 + b + 1;
@@ -287,12 +337,18 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
 }
 
 // 4'. The chunked worker. A tile claims a run of C consecutive operations
-// (lane j holds operation start + j) and works through it batch segment by
-// batch segment, each segment once its batch is open (the frontier rule of
-// the single-op worker). A segment uses the chunk engines of the per-batch
-// kernels: all its keys are routed together (descend_chunk), its inserts and
-// erases go through the batched leaf phase (write_chunk), its reads start
-// from their routed leaves, and its writes are counted with one atomic.
+// (lane j holds operation start + j) and routes all of them at once, before
+// their batches open: every key stops one level above the leaves and takes
+// its leaf from the parent's entry (routing reads only the latest internal
+// nodes, which need no frontier). The run is then worked through batch
+// segment by batch segment, each once its batch is open (the frontier rule of
+// the single-op worker), with only the leaf side left to do: inserts and
+// erases go through the batched leaf phase (write_chunk) from the leaf hints,
+// finds run a pass engine that starts at the hints and moves right or down
+// as the tree has changed since (find_chunk_from), ranges start there one at
+// a time; the segment's writes are counted with one atomic. So the walk from
+// the root, about half of an operation's dependent loads, overlaps the wait
+// for earlier batches instead of following it.
 // C follows the size of the batch the wave claimed from last: batch size /
 // tiles in the grid, clamped to [1, W], so small batches still spread one
 // operation per tile and large ones are taken W at a time. Claims stay in
@@ -301,13 +357,15 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
 // segments are done: the queue drains as in the single-op worker.
 template <int NW>
 __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, batch_job* job) {
+  if (ld32_coherent(&job->bad)) return;  // rejected call: nothing runs
   const tree_view<NW, true> tv(tp);
   using node_t = node<NW, true>;
   tile<NW> t;
-  const unsigned long long nops = job->nops;
-  const uint32_t nb = job->nb, L = job->L, clock0 = job->clock0;
+  const unsigned long long nops = job->nops, first_op = job->first_op;
+  const uint32_t L = job->L, clock0 = job->clock0;
   const op_record* ops = job->ops;
   const unsigned long long* offsets = job->offsets;
+  const uint32_t* op_batch = job->op_batch;
   uint32_t* results = job->results;
   uint64_t* pairs = job->pairs;
   const unsigned long long pair_slots = job->pair_slots;
@@ -324,18 +382,23 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
   // about one right move stale at most).
   __shared__ route_cache<NW, 16384> rc;
   if constexpr (NW == 32)
-    tv.build_route_cache(rc, nops >= 4 * tiles_total ? nops / node_t::kData + 1 : ~0ull);
+    tv.build_route_cache(rc, nops - first_op >= 4 * tiles_total ? (nops - first_op) / node_t::kData + 1 : ~0ull);
   write_ctx ctx = tv.begin_writes(t);
   ctx.exact = 1;
+  ctx.track = 0;  // stamps are fixed per batch: clock0 + b + 1
   int32_t seen = -1;
-  bool have = false;                       // tile holds an unfinished chunk (tile-uniform)
-  unsigned long long cstart = 0, seg = 0, end = 0;  // chunk start, next op, chunk end
-  uint32_t b = 0;                          // batch of op `seg`
-  uint32_t wave_b = 0;                     // batch of the wave's last claim (chunk sizing)
-  op_record op{};                          // lane j: operation cstart + j
+  bool have = false;          // tile holds an unfinished run (tile-uniform)
+  uint64_t pending = 0;       // lanes of the run not yet done (tile-uniform)
+  unsigned long long cstart = 0;
+  uint32_t wave_b = 0;        // batch of the wave's last claim (run sizing)
+  op_record op{};             // lane j: operation cstart + j
+  uint32_t ob = 0;            // lane j: its batch
+  uint32_t hint = kNone, hint_par = kNone;  // lane j: leaf hint and its parent
+  bool routed = false;        // lane j: its key was routed
   uint32_t waits = 0;
   for (;;) {
     const uint64_t idle = __builtin_amdgcn_ballot_w64(!have && t.leader());
+    bool fresh = false;  // tile-uniform: claimed a run in this pass
     if (idle) {
       const uint32_t n_idle = (uint32_t)__builtin_popcountll(idle);
       const uint32_t first = (uint32_t)__builtin_ctzll(idle);
@@ -346,19 +409,44 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
       if (lane == first) base = atomic_add64(&job->next_op, (unsigned long long)n_idle * C);
       base = ((unsigned long long)__builtin_amdgcn_readlane((int)(uint32_t)(base >> 32), (int)first) << 32) |
              (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, (int)first);
-      if (base < nops) wave_b = batch_after(offsets, nb, base, wave_b);
       if (!have) {
         const uint64_t below = (my_tile == 0) ? 0ull : (idle & ((1ull << (my_tile * NW)) - 1));
         cstart = base + (unsigned long long)__builtin_popcountll(below) * C;
         if (cstart >= nops) break;  // queue drained
-        end = cstart + C < nops ? cstart + C : nops;
-        seg = cstart;
-        if (cstart + r < end) op = ops[cstart + r];
-        b = batch_after(offsets, nb, seg, b);
+        const unsigned long long end = cstart + C < nops ? cstart + C : nops;
+        const bool mine = cstart + r < end;
+        if (mine) {
+          op = ops[cstart + r];
+          ob = op_batch[cstart + r - first_op];
+        }
+        pending = t.ballot(mine);
         have = true;
+        fresh = true;
+      }
+      // size the wave's next runs from the batch its claim started in
+      if (base < nops) wave_b = op_batch[base - first_op];
+    }
+    // route the operations just claimed, whatever their batch
+    if (__builtin_amdgcn_ballot_w64(fresh)) {
+      uint32_t rkey = op.key;
+      if (op.type == kOpRange && rkey == 0) rkey = 1;
+      const bool route = fresh && ((pending >> r) & 1) && valid_key(rkey) && !(op.type == kOpRange && op.arg <= rkey);
+      const uint64_t todo = t.ballot(route);
+      uint32_t st0 = kNone, sp0 = kNone;
+      if constexpr (NW == 32) {
+        if (route) st0 = tv.route_lane(rc, rkey, sp0);
+      }
+      uint32_t lf = kNone, pr = kNone;
+      tv.template descend_chunk<true>(rkey, todo, lf, pr, t, todo, st0, sp0);
+      if (fresh) {
+        hint = lf;
+        hint_par = pr;
+        routed = route && lf < tv.cap_;
       }
     }
-    // the segment's batch must be open (all writes of earlier batches done)
+    if (!have) continue;
+    // the run's next segment: its pending operations of the lowest pending batch
+    const uint32_t b = t.shfl(ob, (uint32_t)lowest_bit(pending));
     const bool blocked = (int32_t)b - 1 > seen;
     const uint64_t need = __builtin_amdgcn_ballot_w64(blocked && t.leader());
     if (need) {
@@ -373,7 +461,7 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
         tv.flag(kErrCorrupt, t);
         break;
       }
-      if (__builtin_amdgcn_ballot_w64(!((int32_t)b - 1 > seen)) == 0) {
+      if (__builtin_amdgcn_ballot_w64(have && !((int32_t)b - 1 > seen)) == 0) {
         for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(8);
         if (backoff < 16) backoff *= 2;
       }
@@ -381,76 +469,52 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
     }
     waits = 0;
     backoff = 1;
-    // one segment: the chunk's operations of batch b
-    const unsigned long long seg_end = end < offsets[b + 1] ? end : offsets[b + 1];
-    const bool mine = cstart + r >= seg && cstart + r < seg_end;
+    const uint32_t ts = clock0 + b;
+    const bool mine = ((pending >> r) & 1) && ob == b;
+    const uint64_t seg = t.ballot(mine);
     const bool is_w = mine && (op.type == kOpInsert || op.type == kOpErase);
     const bool is_rd = mine && !is_w;
-    uint32_t rkey = op.key;
-    if (op.type == kOpRange && rkey == 0) rkey = 1;
-    const bool route = mine && valid_key(rkey) && !(op.type == kOpRange && op.arg <= rkey);
-    uint32_t leaf = kNone, par = kNone, res = 0;
-    uint32_t st0 = kNone, sp0 = kNone;
-    if constexpr (NW == 32) {
-      if (route) st0 = tv.route_lane(rc, rkey, sp0);
-    }
-    tv.template descend_chunk<true>(rkey, t.ballot(route), leaf, par, t, t.ballot(is_w), st0, sp0);
-    // writes: batched leaf phase, then the keys it hands back one by one
-    ctx.cur = clock0 + b + 1;
+    uint32_t res = 0;
+    // writes: batched leaf phase from the hints, then the keys it hands back one by one
+    ctx.cur = ts + 1;
     const uint64_t qw = t.ballot(is_w);
     for (int w = 0; w < 2; ++w) {
       uint64_t q = t.ballot(is_w && op.type == (w == 0 ? kOpInsert : kOpErase));
       if (!q) continue;
       ctx.want = (uint32_t)__builtin_popcountll(q);
-      q = (w == 0) ? tv.template write_chunk<0>(op.key, op.arg, q, leaf, res, ctx, t)
-                   : tv.template write_chunk<1>(op.key, 0u, q, leaf, res, ctx, t);
+      q = (w == 0) ? tv.template write_chunk<0>(op.key, op.arg, q, hint, res, ctx, t)
+                   : tv.template write_chunk<1>(op.key, 0u, q, hint, res, ctx, t);
       while (q) {
         const uint32_t j = (uint32_t)lowest_bit(q);
         ctx.want = (uint32_t)__builtin_popcountll(q);
         q &= q - 1;
         const uint32_t k = t.shfl(op.key, j);
-        const uint32_t v = (w == 0) ? tv.template write<0>(k, t.shfl(op.arg, j), ctx, t, t.shfl(leaf, j), t.shfl(par, j))
-                                    : tv.template write<1>(k, 0u, ctx, t, t.shfl(leaf, j), t.shfl(par, j));
+        const uint32_t lf = t.shfl(hint, j), pr = t.shfl(hint_par, j);
+        const uint32_t v = (w == 0) ? tv.template write<0>(k, t.shfl(op.arg, j), ctx, t, lf, pr)
+                                    : tv.template write<1>(k, 0u, ctx, t, lf, pr);
         if (r == j) res = v;
       }
     }
-    // reads at the batch's snapshot, from their routed leaves: finds together,
-    // ranges one by one
-    const uint64_t routed = t.ballot(route);
+    // reads at the batch's snapshot, from the hints: finds together, ranges one by one
+    const uint64_t ok = t.ballot(routed);
     {
       const uint64_t qf = t.ballot(is_rd && op.type == kOpFind);
       if (qf) {
-        const uint32_t v = tv.find_chunk_conc(op.key, leaf, clock0 + b, qf & routed, t);
+        const uint32_t v = tv.find_chunk_from(op.key, hint, ts, qf & ok, t);
         if ((qf >> r) & 1) res = v;
       }
     }
-    uint64_t qr = t.ballot(is_rd && op.type != kOpFind);
+    uint64_t qr = t.ballot(is_rd && op.type == kOpRange);
     while (qr) {
       const uint32_t j = (uint32_t)lowest_bit(qr);
       qr &= qr - 1;
-      const uint32_t ty = t.shfl(op.type, j), k = t.shfl(op.key, j), a = t.shfl(op.arg, j);
-      const uint32_t lf = t.shfl(leaf, j);
-      const bool ok = (routed >> j) & 1;
-      uint32_t v;
-      if (ty == kOpFind) {
-        v = kEmpty;
-        if (ok) {
-          node_t n;
-          n.load(tv.at(lf), t);
-          v = tv.template find_in<true>(k, clock0 + b, n, lf, t);
-        }
-      } else {
-        v = 0;
-        const uint32_t sl = t.shfl(op.slot, j);
-        uint64_t* out = (pairs && sl < pair_slots) ? pairs + (size_t)sl * L : nullptr;
-        if (pairs && sl >= pair_slots) tv.flag(kErrCorrupt, t);
-        if (ok) {
-          node_t n;
-          n.load(tv.at(lf), t);
-          v = tv.template range_in<true>(k == 0 ? 1u : k, a, clock0 + b, n, lf, out, L, t);
-        }
-        if (v > L) tv.flag(kErrRangeOverflow, t);
-      }
+      const uint32_t k = t.shfl(op.key, j), a = t.shfl(op.arg, j), lf = t.shfl(hint, j);
+      const uint32_t sl = t.shfl(op.slot, j);
+      uint64_t* out = (pairs && sl < pair_slots) ? pairs + (size_t)sl * L : nullptr;
+      if (pairs && sl >= pair_slots) tv.flag(kErrCorrupt, t);
+      uint32_t v = 0;
+      if ((ok >> j) & 1) v = tv.template range<true, false>(k == 0 ? 1u : k, a, ts, out, L, t, lf);
+      if (v > L) tv.flag(kErrRangeOverflow, t);
       if (r == j) res = v;
     }
     if (mine) results[cstart + r] = res;
@@ -467,9 +531,8 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
         }
       }
     }
-    seg = seg_end;
-    if (seg >= end) have = false;
-    else b = batch_after(offsets, nb, seg, b);
+    pending &= ~seg;
+    if (!pending) have = false;
   }
   tv.finish(ctx, t);
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -83,12 +84,22 @@ struct mvbt_tree {
   std::map<hipStream_t, hipEvent_t> write_ev;
   hipEvent_t maint_ev = nullptr;
   bool spill_maybe = false;
-  // Batched form: device job descriptor, per-batch scratch, graph.
+  // Launches with launch-fixed write stamps (batched form, mixed calls) run
+  // alone among this handle's writers: they wait for every other stream's
+  // last write, and writes issued later on other streams wait for them.
+  hipEvent_t excl_ev = nullptr;
+  hipStream_t excl_stream = nullptr;
+  bool excl_set = false;
+  // Batched form: device job descriptor, per-batch scratch, conflict table, graph.
   batch_job* job = nullptr;
   uint32_t* b_nwrites = nullptr;
   uint32_t* b_done = nullptr;
   int32_t* b_advance = nullptr;
   uint32_t b_capacity = 0;
+  uint32_t* b_hash = nullptr;
+  uint64_t b_hash_cap = 0;  // entries (a power of two)
+  uint32_t* b_op_batch = nullptr;
+  uint64_t b_op_cap = 0;    // entries
   uint32_t worker_waves = 16;
   uint32_t worker_mode = 1;  // 0: one operation per tile; 1: chunks (batch_chunk_worker_kernel)
   hipGraph_t graph = nullptr;
@@ -297,9 +308,44 @@ mvbt_status clock_advanced(mvbt_tree* t, hipStream_t s, uint32_t by) {
 mvbt_status before_write(mvbt_tree* t, hipStream_t s) {
   t->fresh = false;
   if (mvbt_status rc = after_clock(t, s)) return rc;
+  if (t->excl_set && t->excl_stream != s) MVBT_HIP(hipStreamWaitEvent(s, t->excl_ev, 0));
   if (t->maint_ev) MVBT_HIP(hipStreamWaitEvent(s, t->maint_ev, 0));
   if ((t->spill_maybe || t->external_writers) && !other_writers_busy(t, s))
     return launch_maintenance(t, s, 0);
+  return MVBT_OK;
+}
+
+// Before a launch with launch-fixed write stamps on stream s: it must not
+// overlap another stream's writes (a leaf stamped by them after its own
+// stamps would break the stamp order of version chains), so stream s waits
+// for every other stream's last write.
+mvbt_status isolate(mvbt_tree* t, hipStream_t s) {
+  for (auto& kv : t->write_ev)
+    if (kv.first != s) MVBT_HIP(hipStreamWaitEvent(s, kv.second, 0));
+  return MVBT_OK;
+}
+// After such a launch: writes issued later on other streams wait for it.
+mvbt_status isolated_done(mvbt_tree* t, hipStream_t s) {
+  if (!t->excl_ev) MVBT_HIP(hipEventCreateWithFlags(&t->excl_ev, hipEventDisableTiming));
+  MVBT_HIP(hipEventRecord(t->excl_ev, s));
+  t->excl_stream = s;
+  t->excl_set = true;
+  return MVBT_OK;
+}
+// Snapshot ids a call may still reserve: ids run up to 0xFFFFFFFE
+// (0xFFFFFFFF names the latest version). The host copy of the clock is exact
+// until a device view is handed out; then the device clock is read.
+mvbt_status reserve_ids(mvbt_tree* t, uint64_t count) {
+  uint64_t clock = t->clock;
+  if (t->external_writers) {
+    uint32_t c = 0;
+    MVBT_HIP(hipDeviceSynchronize());
+    MVBT_HIP(hipMemcpy(&c, &t->st->clock, sizeof(c), hipMemcpyDeviceToHost));
+    clock = c;
+  }
+  if (clock + count > (uint64_t)MVBT_LATEST)
+    return fail(MVBT_ERR_UNSUPPORTED, "snapshot ids exhausted: the call needs " + std::to_string(count) +
+                                          " ids, the clock is at " + std::to_string(clock));
   return MVBT_OK;
 }
 
@@ -441,29 +487,58 @@ mvbt_status walk_tree(const host_walk& w, std::vector<mvbt_pair>* pairs, uint64_
 
 // ---- batched form helpers ----------------------------------------------------
 
-mvbt_status ensure_batch_scratch(mvbt_tree* t, uint32_t nb) {
+// Conflict-table entries for a call of n operations: a power of two at least
+// twice n (and at least 1024).
+uint64_t conflict_entries(uint64_t n) {
+  uint64_t e = 1024;
+  while (e < 2 * n) e *= 2;
+  return e;
+}
+
+mvbt_status ensure_batch_scratch(mvbt_tree* t, uint32_t nb, uint64_t nops) {
   if (!t->job) {
     MVBT_HIP(hipMalloc(&t->job, sizeof(batch_job)));
     MVBT_HIP(hipMemset(t->job, 0, sizeof(batch_job)));
   }
-  if (nb <= t->b_capacity) return MVBT_OK;
-  uint32_t cap = 1024;
-  while (cap < nb) cap *= 2;
+  const uint64_t want_hash = conflict_entries(nops);
+  if (nb <= t->b_capacity && want_hash <= t->b_hash_cap && nops <= t->b_op_cap) return MVBT_OK;
   MVBT_HIP(hipDeviceSynchronize());
-  if (t->b_nwrites) (void)hipFree(t->b_nwrites);
-  if (t->b_done) (void)hipFree(t->b_done);
-  if (t->b_advance) (void)hipFree(t->b_advance);
-  t->b_nwrites = nullptr;
-  t->b_done = nullptr;
-  t->b_advance = nullptr;
-  t->b_capacity = 0;
-  MVBT_HIP(hipMalloc(&t->b_nwrites, cap * sizeof(uint32_t)));
-  MVBT_HIP(hipMalloc(&t->b_done, cap * sizeof(uint32_t)));
-  MVBT_HIP(hipMalloc(&t->b_advance, cap * sizeof(int32_t)));
+  if (nb > t->b_capacity) {
+    uint32_t cap = 1024;
+    while (cap < nb) cap *= 2;
+    if (t->b_nwrites) (void)hipFree(t->b_nwrites);
+    if (t->b_done) (void)hipFree(t->b_done);
+    if (t->b_advance) (void)hipFree(t->b_advance);
+    t->b_nwrites = nullptr;
+    t->b_done = nullptr;
+    t->b_advance = nullptr;
+    t->b_capacity = 0;
+    MVBT_HIP(hipMalloc(&t->b_nwrites, cap * sizeof(uint32_t)));
+    MVBT_HIP(hipMalloc(&t->b_done, cap * sizeof(uint32_t)));
+    MVBT_HIP(hipMalloc(&t->b_advance, cap * sizeof(int32_t)));
+    t->b_capacity = cap;
+  }
+  if (want_hash > t->b_hash_cap) {
+    if (t->b_hash) (void)hipFree(t->b_hash);
+    t->b_hash = nullptr;
+    t->b_hash_cap = 0;
+    MVBT_HIP(hipMalloc(&t->b_hash, want_hash * sizeof(uint32_t)));
+    t->b_hash_cap = want_hash;
+  }
+  if (nops > t->b_op_cap) {
+    uint64_t cap = 1024;
+    while (cap < nops) cap *= 2;
+    if (t->b_op_batch) (void)hipFree(t->b_op_batch);
+    t->b_op_batch = nullptr;
+    t->b_op_cap = 0;
+    MVBT_HIP(hipMalloc(&t->b_op_batch, cap * sizeof(uint32_t)));
+    t->b_op_cap = cap;
+  }
   // The scratch pointers live in the device descriptor; per-call parameters do not.
-  uint32_t* ptrs[3] = {t->b_nwrites, t->b_done, reinterpret_cast<uint32_t*>(t->b_advance)};
+  uint32_t* ptrs[5] = {t->b_nwrites, t->b_done, reinterpret_cast<uint32_t*>(t->b_advance), t->b_hash,
+                       t->b_op_batch};
+  static_assert(offsetof(batch_job, op_batch) == offsetof(batch_job, nwrites) + 4 * sizeof(void*), "scratch layout");
   MVBT_HIP(hipMemcpy(&t->job->nwrites, ptrs, sizeof(ptrs), hipMemcpyHostToDevice));
-  t->b_capacity = cap;
   return MVBT_OK;
 }
 
@@ -485,7 +560,7 @@ void destroy_graph(mvbt_tree* t) {
   t->graph_setup = nullptr;
 }
 
-// Explicit five-node graph: setup -> count -> frontier -> worker -> finish.
+// Explicit six-node graph: setup -> clear -> count -> frontier -> worker -> finish.
 // Every node but setup reads its inputs from the device descriptor, so a call
 // only updates the setup node's parameters.
 mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
@@ -509,9 +584,11 @@ mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
     prev = node;
     return e;
   };
+  void* count_args[] = {&job, &st};
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_setup_kernel), 1, 64, setup_args));
   t->graph_setup = node;
-  MVBT_HIP(add(reinterpret_cast<void*>(&batch_count_kernel), t->num_cus * 4, kBlock, job_args));
+  MVBT_HIP(add(reinterpret_cast<void*>(&batch_clear_kernel), t->num_cus * 4, kBlock, job_args));
+  MVBT_HIP(add(reinterpret_cast<void*>(&batch_count_kernel), t->num_cus * 4, kBlock, count_args));
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_frontier_kernel), t->num_cus, kBlock, job_args));
   tree_ptrs tp = t->ptrs();
   void* worker_args[] = {&tp, &job};
@@ -538,8 +615,11 @@ mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
   if (cfg.pool_nodes == 0) cfg.pool_nodes = 1ull << 22;
   if (cfg.pool_nodes < 4 || cfg.pool_nodes >= kMaxNodes)
     return fail(MVBT_ERR_INVALID_ARG, "pool_nodes must be in [4, 2^29-1)");
-  if (cfg.retire_capacity == 0) cfg.retire_capacity = 1ull << 22;
-  if (!is_pow2(cfg.retire_capacity)) return fail(MVBT_ERR_INVALID_ARG, "retire_capacity must be a power of two");
+  if (cfg.retire_capacity != 0 && !is_pow2(cfg.retire_capacity))
+    return fail(MVBT_ERR_INVALID_ARG, "retire_capacity must be a power of two");
+  // Every retired copy awaiting reclamation occupies a pool slot, so a ring
+  // of at least pool_nodes entries never fills (a full ring would leak).
+  while (cfg.retire_capacity < cfg.pool_nodes) cfg.retire_capacity = cfg.retire_capacity ? cfg.retire_capacity * 2 : 1;
 
   int dev = cfg.device;
   if (dev < 0) MVBT_HIP(hipGetDevice(&dev));
@@ -618,6 +698,9 @@ mvbt_status mvbt_destroy(mvbt_tree* t) {
   if (t->b_nwrites) (void)hipFree(t->b_nwrites);
   if (t->b_done) (void)hipFree(t->b_done);
   if (t->b_advance) (void)hipFree(t->b_advance);
+  if (t->b_hash) (void)hipFree(t->b_hash);
+  if (t->b_op_batch) (void)hipFree(t->b_op_batch);
+  if (t->excl_ev) (void)hipEventDestroy(t->excl_ev);
   delete t;
   return MVBT_OK;
 }
@@ -897,12 +980,15 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
   if (num_ops == 0 || num_batches == 0) return MVBT_OK;
   if (!ops || !batch_offsets || !results) return fail(MVBT_ERR_INVALID_ARG, "null ops, offsets or results");
   if (num_batches > (1u << 30)) return fail(MVBT_ERR_INVALID_ARG, "too many batches");
+  if (num_ops >= (1ull << 31)) return fail(MVBT_ERR_INVALID_ARG, "num_ops must be below 2^31");
   if (pairs && L == 0) return fail(MVBT_ERR_INVALID_ARG, "L must be > 0 when pairs is given");
   device_guard g(t->device);
   std::lock_guard<std::mutex> lk(t->mu);
-  if (mvbt_status rc = ensure_batch_scratch(t, num_batches)) return rc;
+  if (mvbt_status rc = reserve_ids(t, num_batches)) return rc;
+  if (mvbt_status rc = ensure_batch_scratch(t, num_batches, num_ops)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (mvbt_status rc = before_write(t, s)) return rc;
+  if (mvbt_status rc = isolate(t, s)) return rc;
   batch_params p{};
   p.ops = reinterpret_cast<const op_record*>(ops);
   p.nops = num_ops;
@@ -913,6 +999,7 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
   p.pairs = reinterpret_cast<uint64_t*>(pairs);
   p.pair_slots = pairs ? pair_slots : 0;
   p.base_out = snapshot_base;
+  p.hash_mask = (uint32_t)(conflict_entries(num_ops) - 1);
   if (flags & MVBT_USE_GRAPH) {
     if (!t->graph_exec) {
       if (mvbt_status rc = build_graph(t, p)) return rc;
@@ -929,7 +1016,8 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
     MVBT_HIP(hipGraphLaunch(t->graph_exec, s));
   } else {
     hipLaunchKernelGGL(batch_setup_kernel, dim3(1), dim3(64), 0, s, t->job, t->st, p);
-    hipLaunchKernelGGL(batch_count_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job);
+    hipLaunchKernelGGL(batch_clear_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job);
+    hipLaunchKernelGGL(batch_count_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job, t->st);
     hipLaunchKernelGGL(batch_frontier_kernel, dim3(t->num_cus), dim3(kBlock), 0, s, t->job);
     tree_ptrs tp = t->ptrs();
     batch_job* job = t->job;
@@ -940,6 +1028,7 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
   }
   MVBT_HIP(hipGetLastError());
   if (mvbt_status rc = clock_advanced(t, s, num_batches)) return rc;
+  if (mvbt_status rc = isolated_done(t, s)) return rc;
   return after_write(t, s);
 }
 
@@ -1011,14 +1100,18 @@ mvbt_status mvbt_concurrent_insert_range(mvbt_tree* t, const uint32_t* keys, con
   if (num_ranges && result && L == 0) return fail(MVBT_ERR_INVALID_ARG, "L must be > 0 when result is given");
   device_guard g(t->device);
   std::lock_guard<std::mutex> lk(t->mu);
+  if (t->versioned)
+    if (mvbt_status rc = reserve_ids(t, 1)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (mvbt_status rc = before_write(t, s)) return rc;
+  if (mvbt_status rc = isolate(t, s)) return rc;
   if (mvbt_status rc = table_for(t->nw, t->versioned)
                            ->insert_range(t, keys, values, num_insertions, lower, upper, num_ranges, nullptr,
                                           reinterpret_cast<uint64_t*>(result), counts, L, s))
     return rc;
   if (t->versioned)  // the call reserved one snapshot id
     if (mvbt_status rc = clock_advanced(t, s, 1)) return rc;
+  if (mvbt_status rc = isolated_done(t, s)) return rc;
   return after_write(t, s);
 }
 
@@ -1031,14 +1124,18 @@ mvbt_status mvbt_concurrent_find_erase(mvbt_tree* t, const uint32_t* find_keys, 
   if (num_erasures && !erase_keys) return fail(MVBT_ERR_INVALID_ARG, "null erase keys");
   device_guard g(t->device);
   std::lock_guard<std::mutex> lk(t->mu);
+  if (t->versioned)
+    if (mvbt_status rc = reserve_ids(t, 1)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (mvbt_status rc = before_write(t, s)) return rc;
+  if (mvbt_status rc = isolate(t, s)) return rc;
   if (mvbt_status rc = table_for(t->nw, t->versioned)
                            ->find_erase(t, erase_keys, nullptr, num_erasures, find_keys, nullptr, num_finds,
                                         find_results, nullptr, nullptr, 0, s))
     return rc;
   if (t->versioned)  // the call reserved one snapshot id
     if (mvbt_status rc = clock_advanced(t, s, 1)) return rc;
+  if (mvbt_status rc = isolated_done(t, s)) return rc;
   return after_write(t, s);
 }
 

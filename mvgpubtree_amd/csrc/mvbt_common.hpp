@@ -52,6 +52,9 @@ enum : uint32_t {
   kErrRangeOverflow = 1u << 4,   // a range query produced more than L pairs
   kErrCorrupt = 1u << 5,         // a pointer left the pool or a loop exceeded its bound
   kErrSnapshotId = 1u << 6,      // a snapshot id returned without waiting differs from the clock
+  kErrStampOrder = 1u << 7,      // a fixed-stamp write met a leaf stamped after it (overlapping writers)
+  kErrBadBatch = 1u << 8,        // a batched call was rejected: a batch writes one key twice, an
+                                 // unknown op type, or offsets that disagree with the op count
 };
 
 // Loop bounds: every device loop terminates even on a corrupted structure.
@@ -90,6 +93,12 @@ MVBT_DEV uint32_t atomic_and32(uint32_t* p, uint32_t v) {
 }
 MVBT_DEV uint32_t atomic_add32(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Compare-and-swap; returns the value found (== expected on success).
+MVBT_DEV uint32_t atomic_cas32(uint32_t* p, uint32_t expected, uint32_t desired) {
+  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return expected;
 }
 MVBT_DEV unsigned long long atomic_add64(unsigned long long* p, unsigned long long v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

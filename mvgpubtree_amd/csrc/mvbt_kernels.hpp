@@ -134,7 +134,6 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
   write_ctx ctx = tv.begin_writes(t);
   ctx.exact = 1;  // ctx.want below is the chunk's exact remaining count
   for (uint64_t base = c.first; base < n; base += c.stride) {
-    if (base != c.first) ctx.cur = tv.refresh_stamp(t);
     uint64_t i = base + t.rank();
     bool active = i < n;
     uint32_t key = active ? keys[i] : 0u;
@@ -197,6 +196,7 @@ __global__ void __launch_bounds__(256)
   write_ctx ctx = tv.begin_writes(t);
   ctx.exact = 1;
   if (VER) ctx.cur = ts + 1;
+  ctx.track = 0;  // the call's writes are stamped right after its snapshot
   bool overflow = false;
   for (uint64_t base = c.first; base < n; base += c.stride) {
     const uint64_t i = base + t.rank();
@@ -231,9 +231,9 @@ __global__ void __launch_bounds__(256)
         if constexpr (IR) {
           const uint64_t rj = base + j - nw;
           uint64_t* slot = pairs ? pairs + rj * (uint64_t)L : nullptr;
-          r = tv.template range<true>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t);
+          r = tv.template range<true, false>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t);
         } else {
-          r = tv.template find<true>(t.shfl(lo, j), ts, t);
+          r = tv.template find<true, false>(t.shfl(lo, j), ts, t);
         }
         if (t.rank() == j) result = r;
       }
@@ -282,8 +282,11 @@ __global__ void reclaim_prep_kernel(tree_state* st, unsigned long long end_pos,
   unsigned long long end = end_pos < tail ? end_pos : tail;
   if (end < head) end = head;
   unsigned long long count = end - head;
-  // Positions at or beyond head + cap overflowed and were never stored.
-  if (count > st->retire_cap) count = st->retire_cap;
+  // The ring holds at least as many entries as the pool has nodes, so it
+  // cannot fill; were it ever to, the positions past head + cap were never
+  // stored and their slots still name nodes already freed: free nothing
+  // from the ring then (those copies leak instead of being freed twice).
+  if (count > st->retire_cap || (st->error_flags & kErrRetireOverflow)) count = 0;
   long long top = st->free_top;
   unsigned long long base = top > 0 ? (unsigned long long)top : 0ull;
   unsigned long long spilled = st->spill_count;

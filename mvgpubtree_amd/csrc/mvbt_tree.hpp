@@ -85,6 +85,8 @@ struct write_ctx {
   uint32_t mag_n;  // tile-uniform
   uint32_t ret_n;  // tile-uniform
   uint32_t cur;    // write stamp (tile-uniform)
+  uint32_t track;  // nonzero: the stamp follows the clock, read again under each leaf lock;
+                   // zero: a launch-fixed stamp (batched worker, mixed calls)
   uint32_t want;   // writes this tile still has queued (caps refills)
   uint32_t grow;   // next refill size: starts at 2, doubles up to W
   uint32_t exact;  // nonzero: `want` is exact, version-copy refills take want + 2
@@ -163,17 +165,38 @@ struct tree_view : tree_ptrs {
     c.mag_n = 0;
     c.ret_n = 0;
     c.cur = refresh_stamp(t);
+    c.track = 1;
     c.want = NW;
     c.grow = 2;
     c.exact = 0;
     c.backoff = 1;
     return c;
   }
-  // Write stamp, read once per chunk of operations instead of once per op.
+  // The clock (leader load, broadcast).
   MVBT_DEV uint32_t refresh_stamp(const tile_t& t) const {
     uint32_t c = 0;
     if (VER && t.leader()) c = ld32_coherent(&st->clock);
     return t.shfl(c, 0);
+  }
+  // Stamp of a write to leaves this tile has just locked. A tile that tracks
+  // the clock reads it only once it holds the locks: every snapshot taken
+  // before then is older than the stamp (the write is invisible to it, its
+  // leaf gets a version copy), and a snapshot taken later was taken after
+  // the lock. A leaf is therefore rewritten in place (stamp == cur) only by
+  // a writer that locked it before any snapshot that can read this content
+  // existed; concurrent snapshot readers rely on that (to_version).
+  MVBT_DEV void stamp_under_lock(write_ctx& c, const tile_t& t) const {
+    if (VER && c.track) c.cur = refresh_stamp(t);
+  }
+  // True when a leaf with version word `ver` must be copied out before a
+  // write stamped c.cur changes it: its content belongs to an older epoch.
+  // A leaf stamped after c.cur (a launch-fixed stamp overlapping another
+  // writer) is written in place under its newer stamp, so every version
+  // chain stays ordered by stamp, and the overlap is flagged.
+  MVBT_DEV bool needs_copy(uint64_t ver, const write_ctx& c, const tile_t& t) const {
+    const uint32_t s = node_t::stamp(ver);
+    if (s > c.cur) flag(kErrStampOrder, t);
+    return s < c.cur;
   }
 
   // ---- allocation (roles of the reference's device_bump_allocator.hpp:45-54
@@ -329,7 +352,18 @@ struct tree_view : tree_ptrs {
 
   // Moves `n` (the leaf stored at `idx`) to its newest version stamped <= ts.
   // Returns false if the key range did not exist at ts (version gap).
-  template <bool CONC>
+  // CONC (writers may run): a live leaf whose stamp is visible is accepted
+  // only if its version word is unchanged when read again after the leaf
+  // read returned. Writers that copy publish the new version word before
+  // they touch the data, so any such write overlapping the read is seen.
+  // GUARD (writers of other launches may run): a leaf may also be rewritten
+  // in place, without a new version word, by a writer that locked it before
+  // the snapshot existed (stamp_under_lock); its content is trusted only
+  // when read after a completed read showed the leaf unlocked. Without
+  // GUARD the caller guarantees that no in-place writer of a visible epoch
+  // runs (the batched worker's frontier, the mixed calls' single snapshot).
+  // ts == kLatest is handled by the callers (see find_in).
+  template <bool CONC, bool GUARD = CONC>
   MVBT_DEV bool to_version(node_t& n, uint32_t idx, uint32_t ts, const tile_t& t) const {
     if constexpr (!VER) {
       return true;
@@ -338,17 +372,21 @@ struct tree_view : tree_ptrs {
       uint64_t ver = n.version(t);
       if (node_t::stamp(ver) <= ts) {
         if constexpr (!CONC) return true;
-        // Validate: the leaf's version word must not have moved while we read it.
+        bool clear = !GUARD;  // GUARD: `n` was read after the leaf was seen unlocked
         for (uint32_t step = 0;; ++step) {
           if (step >= kMaxReadSteps) {
             flag(kErrCorrupt, t);
             return false;
           }
-          uint64_t ver2 = reread_version(idx, t);
-          if (ver2 == ver) return true;
-          ver = ver2;
-          if (node_t::stamp(ver) > ts) break;
-          // Stamp still visible but the word moved: read the leaf again.
+          if (!clear) {
+            clear = !node_t::is_locked(n.meta(t));
+            if (!clear) __builtin_amdgcn_s_sleep(2);
+          } else {
+            const uint64_t ver2 = reread_version(idx, t);
+            if (ver2 == ver) return true;
+            clear = !GUARD;
+          }
+          // read the leaf again (after the observation above returned)
           load<true>(n, idx, t);
           ver = n.version(t);
           if (node_t::stamp(ver) > ts) break;
@@ -603,21 +641,27 @@ struct tree_view : tree_ptrs {
     return res;
   }
 
-  // Concurrent finds of a chunk at snapshot `ts` (versioned trees; writers may
-  // run), each key starting from its routed leaf (lane j: key, leaf). Pass-
-  // based like read_chunk: every pass loads one node for each unfinished key
-  // of a group. A live leaf whose stamp is visible gives a provisional answer
-  // that the next pass confirms by reading the leaf again and comparing the
-  // version words (the validation of to_version<true>); a leaf whose stamp is
-  // newer than ts hands over to its version chain, whose copies are immutable
-  // once published. Returns the value or kEmpty.
-  MVBT_DEV uint32_t find_chunk_conc(uint32_t key, uint32_t leaf, uint32_t ts, uint64_t todo,
+  // Concurrent finds of a chunk at snapshot `ts` (versioned trees; writers
+  // stamped after ts may run, as inside the batched form), each key starting
+  // from a node that is at or left of its leaf, or above it (lane j: key,
+  // start): a leaf hint taken from a parent entry before the batch opened,
+  // possibly stale. Pass-based like read_chunk: every pass loads one node for
+  // each unfinished key of a group. On the latest tree a key moves right
+  // while it is at or above the node's high key and descends through internal
+  // nodes, which finds the leaf covering it now; that leaf's version at ts
+  // covers it at ts (keys only move right). A live leaf whose stamp is
+  // visible gives a provisional answer that the next pass confirms by reading
+  // the leaf again and comparing the version words (writers that copy publish
+  // a new version word before touching the data); a leaf stamped after ts
+  // hands over to its version chain, whose copies are immutable once
+  // published. Returns the value or kEmpty.
+  MVBT_DEV uint32_t find_chunk_from(uint32_t key, uint32_t start, uint32_t ts, uint64_t todo,
                                     const tile_t& t) const {
-    enum : uint32_t { kLive = 0, kCheck = 1, kChain = 2 };
+    enum : uint32_t { kWalk = 0, kCheck = 1, kChain = 2 };
     const uint32_t r = t.rank();
-    uint32_t cur = leaf, phase = kLive, res = kEmpty, seen_ver_lo = 0, seen_ver_hi = 0, prov = kEmpty;
+    uint32_t cur = start, phase = kWalk, res = kEmpty, seen_ver_lo = 0, seen_ver_hi = 0, prov = kEmpty;
     uint32_t err = 0;
-    todo &= t.ballot(valid_key(key) && leaf < cap_);
+    todo &= t.ballot(valid_key(key) && start < cap_);
     for (int base = 0; base < NW; base += kReadGroup) {
       uint64_t active = (todo >> base) & ((kReadGroup == 64) ? ~0ull : ((1ull << kReadGroup) - 1));
       for (uint32_t pass = 0; active != 0; ++pass) {
@@ -638,6 +682,8 @@ struct tree_view : tree_ptrs {
           node_t n;
           n.p = p[j];
           const uint64_t ver = n.version(t);
+          const uint64_t m = n.meta(t);
+          const uint32_t k = t.shfl(key, lj);
           uint32_t ph = t.shfl(phase, lj);
           bool done = false;
           uint32_t next = t.shfl(cur, lj), out = kEmpty, keep_prov = kEmpty;
@@ -647,27 +693,40 @@ struct tree_view : tree_ptrs {
               out = t.shfl(prov, lj);
               done = true;
             } else {
-              ph = kLive;  // the leaf moved on: treat this read as a fresh one
+              ph = kWalk;  // the leaf moved on: treat this read as a fresh one
             }
           }
-          if (!done && (ph == kLive || ph == kChain)) {
-            if (node_t::stamp(ver) > ts) {
+          if (!done && ph == kWalk) {
+            if (k >= node_t::high_key(m)) {
+              next = node_t::sibling(m);
+            } else if (!node_t::is_leaf(m)) {
+              next = n.child_for(k, t);
+            } else if (node_t::stamp(ver) > ts) {
               next = node_t::older(ver);
               ph = kChain;
               if (next == kNone) {
                 err |= kErrVersionGap;
                 done = true;
-              } else if (next >= cap_) {
-                err |= kErrCorrupt;
-                done = true;
               }
-            } else if (ph == kChain) {
-              out = n.lookup(t.shfl(key, lj), t);
-              done = true;
             } else {  // visible live leaf: answer provisionally, confirm next pass
-              keep_prov = n.lookup(t.shfl(key, lj), t);
+              keep_prov = n.lookup(k, t);
               ph = kCheck;
             }
+          } else if (!done && ph == kChain) {
+            if (node_t::stamp(ver) > ts) {
+              next = node_t::older(ver);
+              if (next == kNone) {
+                err |= kErrVersionGap;
+                done = true;
+              }
+            } else {
+              out = n.lookup(k, t);
+              done = true;
+            }
+          }
+          if (!done && next >= cap_) {
+            err |= kErrCorrupt;
+            done = true;
           }
           if (r == lj) {
             if (done) {
@@ -810,19 +869,21 @@ struct tree_view : tree_ptrs {
   }
 
   // ---- find -----------------------------------------------------------------
-  template <bool CONC>
+  template <bool CONC, bool GUARD = CONC>
   MVBT_DEV uint32_t find(uint32_t key, uint32_t ts, const tile_t& t, uint32_t start = kNone) const {
     if (!valid_key(key)) return kEmpty;
     node_t n;
     uint32_t idx = descend<CONC>(key, n, t, nullptr, start);
-    return find_in<CONC>(key, ts, n, idx, t);
+    return find_in<CONC, GUARD>(key, ts, n, idx, t);
   }
-  // Plain trees rewrite a leaf in place (pairs shift on insert and erase), a
-  // multi-lane store a concurrent reader can observe half done; versioned
-  // leaves are validated through their version word instead. So a concurrent
-  // reader of a plain leaf reads it under the leaf's lock: one attempt per
-  // pass of the caller's loop, never a spin of its own, because the holder may
-  // be another tile of the same wave, which releases within the same pass.
+  // A leaf's latest content is rewritten in place (pairs shift on insert and
+  // erase; a versioned leaf keeps one version word per epoch), a multi-lane
+  // store a concurrent reader can observe half done, and no version word
+  // tells the reader. So a concurrent reader of the latest state (every read
+  // of a plain tree, ts == kLatest on a versioned one) reads a leaf under its
+  // lock: one attempt per pass of the caller's loop, never a spin of its own,
+  // because the holder may be another tile of the same wave, which releases
+  // within the same pass. Snapshot reads are validated instead (to_version).
   MVBT_DEV bool lock_and_load(uint32_t idx, node_t& n, const tile_t& t) const {
     if (!try_lock(idx, t)) {
       for (uint32_t i = 0; i < 4; ++i) __builtin_amdgcn_s_sleep(2);
@@ -833,9 +894,9 @@ struct tree_view : tree_ptrs {
   }
 
   // The find once `key` has been routed to leaf `idx`, loaded in `n`.
-  template <bool CONC>
+  template <bool CONC, bool GUARD = CONC>
   MVBT_DEV uint32_t find_in(uint32_t key, uint32_t ts, node_t& n, uint32_t idx, const tile_t& t) const {
-    if constexpr (!VER && CONC) {
+    if (CONC && (!VER || ts == kLatest)) {
       for (uint32_t step = 0; idx < cap_ && step < kMaxReadSteps; ++step) {
         if (!lock_and_load(idx, n, t)) continue;  // busy: try again next pass
         const uint64_t m = n.meta(t);
@@ -851,29 +912,29 @@ struct tree_view : tree_ptrs {
       if (idx != kNone) flag(kErrCorrupt, t);
       return kEmpty;
     } else {
-      if (idx == kNone || !to_version<CONC>(n, idx, ts, t)) return kEmpty;
+      if (idx == kNone || !to_version<CONC, GUARD>(n, idx, ts, t)) return kEmpty;
       return n.lookup(key, t);
     }
   }
 
   // ---- range query [lo, hi) ------------------------------------------------
   // Writes min(count, cap) pairs to out[0..) and returns the true count.
-  template <bool CONC>
+  template <bool CONC, bool GUARD = CONC>
   MVBT_DEV uint32_t range(uint32_t lo, uint32_t hi, uint32_t ts, uint64_t* out, uint32_t cap,
                           const tile_t& t, uint32_t start = kNone) const {
     if (lo == 0) lo = 1;
     if (hi <= lo) return 0;
     node_t n;
     uint32_t idx = descend<CONC>(lo, n, t, nullptr, start);
-    return range_in<CONC>(lo, hi, ts, n, idx, out, cap, t);
+    return range_in<CONC, GUARD>(lo, hi, ts, n, idx, out, cap, t);
   }
   // The range query once `lo` (>= 1, below hi) has been routed to leaf `idx`,
   // loaded in `n`.
-  template <bool CONC>
+  template <bool CONC, bool GUARD = CONC>
   MVBT_DEV uint32_t range_in(uint32_t lo, uint32_t hi, uint32_t ts, node_t& n, uint32_t idx, uint64_t* out,
                              uint32_t cap, const tile_t& t) const {
-    if constexpr (!VER && CONC) {
-      // plain leaves read under lock, one at a time (see lock_and_load)
+    if (CONC && (!VER || ts == kLatest)) {
+      // latest leaves read under lock, one at a time (see lock_and_load)
       uint32_t count = 0;
       bool first = true;
       for (uint32_t step = 0; idx < cap_ && step < kMaxReadSteps; ++step) {
@@ -893,7 +954,7 @@ struct tree_view : tree_ptrs {
       if (idx != kNone) flag(kErrCorrupt, t);
       return count;
     }
-    if (idx == kNone || !to_version<CONC>(n, idx, ts, t)) return 0;
+    if (idx == kNone || !to_version<CONC, GUARD>(n, idx, ts, t)) return 0;
     uint32_t count = 0;
     for (uint32_t step = 0;; ++step) {
       count += n.collect(lo, hi, out, count, cap, t);
@@ -906,7 +967,7 @@ struct tree_view : tree_ptrs {
       }
       idx = sib;
       load<CONC>(n, idx, t);
-      if (!to_version<CONC>(n, idx, ts, t)) break;
+      if (!to_version<CONC, GUARD>(n, idx, ts, t)) break;
     }
     return count;
   }
@@ -1018,7 +1079,7 @@ struct tree_view : tree_ptrs {
   MVBT_DEV bool open_epoch(node_t& n, uint32_t idx, write_ctx& c, const tile_t& t) const {
     if constexpr (VER) {
       uint64_t ver = n.version(t);
-      if (node_t::stamp(ver) != c.cur) {
+      if (needs_copy(ver, c, t)) {
         uint32_t cp = alloc(c, t, true);
         if (cp == kNone) return false;
         n.store_unlocked_copy(at(cp), t);
@@ -1065,7 +1126,7 @@ struct tree_view : tree_ptrs {
     uint32_t y = alloc(c, t);
     bool copied = false;
     if constexpr (VER) {
-      if (y != kNone && node_t::stamp(n.version(t)) != c.cur) {  // new epoch (as open_epoch)
+      if (y != kNone && needs_copy(n.version(t), c, t)) {  // new epoch (as open_epoch)
         const uint32_t cp = alloc(c, t, true);
         if (cp == kNone) {
           y = kNone;  // an allocated y is simply never linked; counted as leaked
@@ -1169,6 +1230,7 @@ struct tree_view : tree_ptrs {
         continue;  // re-read the same leaf
       }
       n.load(at(idx), t);
+      stamp_under_lock(c, t);
       m = n.meta(t);
       if (key >= node_t::high_key(m)) {
         unlock(idx, t);
@@ -1255,6 +1317,7 @@ struct tree_view : tree_ptrs {
       const uint64_t locked = lead & t.ballot(!(old & kLockBit));
       for (uint64_t m = lead & ~locked; m; m &= m - 1)
         left |= todo & t.ballot(my_lead == (uint32_t)lowest_bit(m));
+      if (locked) stamp_under_lock(c, t);  // one clock read covers every leaf locked above
       uint64_t pending = locked, prev = 0;
       for (;;) {
         uint64_t got = 0;
@@ -1324,7 +1387,7 @@ struct tree_view : tree_ptrs {
             if (chg) changed |= 1u << i;
             node_t o;
             o.p = img[i];
-            if (VER && chg && node_t::stamp(o.version(t)) != c.cur) need_copy |= 1u << i;
+            if (VER && chg && needs_copy(o.version(t), c, t)) need_copy |= 1u << i;
           }
         }
         // 4. version copies (new epoch), then their version words

@@ -17,7 +17,8 @@ open-loop (Poisson). Two servers answer the same request stream:
   stream-ordered statistics after each step's reclamation.
 * ``PerRequestServer``: the plain per-request launch path - for each
   request, take a snapshot, launch find and range_query at it, then insert,
-  release, one request at a time (the reference host API's pattern).
+  release, one request at a time (the reference host API's pattern), from a
+  C++ loop over the C ABI (drivers.PerBatchRunner.run_requests).
 
 Both produce identical results for the same request order (a request's
 reads see exactly the requests before it).
@@ -189,30 +190,23 @@ class ContinuousBatchingServer:
 
 
 class PerRequestServer:
-    """Plain per-request launch path."""
+    """Plain per-request launch path: each request is issued at its arrival
+    (or when the previous one completes) and waited for, from C++."""
 
-    def __init__(self, tree, reqs: RequestStream, dev, reclaim_every: int = 64):
+    def __init__(self, tree, reqs: RequestStream, dev, reclaim_every: int = 64, keep_pairs: bool = False):
         self.tree, self.reqs = tree, reqs
         self.runner = drivers.PerBatchRunner(reqs.batches, reqs.range_capacity, dev,
-                                             reclaim_every=reclaim_every)
+                                             reclaim_every=reclaim_every, keep_pairs=keep_pairs)
 
     def run(self):
         import torch
-        tree, R = self.tree, len(self.reqs.batches)
+        R = len(self.reqs.batches)
         arrivals = self.reqs.arrivals
-        done_at = np.zeros(R)
         stream = torch.cuda.current_stream()
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(R):
-            now = time.perf_counter() - t0
-            if arrivals[i] > now:
-                time.sleep(arrivals[i] - now)
-            self.runner.run_one(tree, i, stream)
-            stream.synchronize()
-            done_at[i] = time.perf_counter() - t0
-        total = time.perf_counter() - t0
+        done_at = self.runner.run_requests(self.tree, arrivals, stream)
+        total = float(done_at[-1]) if R else 0.0
         lat = done_at - arrivals
         return {"requests": R, "ops": self.reqs.num_ops, "steps": R,
-                "Mops": round(self.reqs.num_ops / total / 1e6, 3), "wall_s": round(total, 4),
+                "Mops": round(self.reqs.num_ops / total / 1e6, 3) if total else 0.0, "wall_s": round(total, 4),
                 **_percentiles(lat)}, lat

@@ -50,6 +50,8 @@ def test_header_constants_match_mirror():
     assert int(consts["MVBT_FLAG_RANGE_OVERFLOW"], 16) == mv.FLAG_RANGE_OVERFLOW
     assert int(consts["MVBT_FLAG_CORRUPT"], 16) == mv.FLAG_CORRUPT
     assert int(consts["MVBT_FLAG_SNAPSHOT_ID"], 16) == mv.FLAG_SNAPSHOT_ID
+    assert int(consts["MVBT_FLAG_STAMP_ORDER"], 16) == mv.FLAG_STAMP_ORDER
+    assert int(consts["MVBT_FLAG_BAD_BATCH"], 16) == mv.FLAG_BAD_BATCH
 
 
 def test_device_error_bits_match_header_flags():

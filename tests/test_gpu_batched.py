@@ -83,8 +83,9 @@ def test_chunked_worker_multi_op_chunks(dev, waves, nw):
 def test_per_batch_path_matches_oracle(dev):
     w = small_config4(seed=7, num_batches=60)
     tree = build_tree(w, dev)
-    r = drivers.PerBatchRunner(w.batches, w.range_capacity, dev)
-    got = r.run(tree, collect=True)
+    r = drivers.PerBatchRunner(w.batches, w.range_capacity, dev, keep_pairs=True)
+    r.run(tree)
+    got = [r.batch_results(i) for i in range(len(w.batches))]
     st = oracle.SortedState(w.initial_keys, w.initial_values)
     for (f, c, p), b in zip(got, w.batches):
         assert np.array_equal(f, st.find(b.find_keys))

@@ -48,8 +48,10 @@ def test_served_results_equal_sequential_requests(dev, max_inflight, use_graph):
 
 def test_per_request_path_matches_oracle(dev):
     init, reqs, tree = setup(dev, seed=5, n_requests=80)
-    srv = serving.PerRequestServer(tree, reqs, dev)
-    got = [srv.runner.run_one(tree, i, collect=True) for i in range(len(reqs.batches))]
+    srv = serving.PerRequestServer(tree, reqs, dev, keep_pairs=True)
+    summary, lat = srv.run()
+    assert summary["requests"] == len(reqs.batches) and np.all(lat >= 0)
+    got = [srv.runner.batch_results(i) for i in range(len(reqs.batches))]
     st = oracle.SortedState(init, wl.value_of(init))
     for (f, c, p), b in zip(got, reqs.batches):
         assert np.array_equal(f, st.find(b.find_keys))
