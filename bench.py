@@ -296,11 +296,11 @@ def run_config2(args):
         # (batched form vs plain per-batch launches) and config 5 (served form
         # vs plain per-request launches, p50/p99 latency) at full size
         w4 = wl.config4(seed=4, n_initial=n_initial, num_batches=4096)
-        c4 = batched_vs_plain(w4, w4.range_capacity, dev, args.c4_node_width, 1, args.c4_waves)
+        c4 = batched_vs_plain(w4, w4.range_capacity, dev, args.c4_node_width, args.c4_mode, args.c4_waves)
         del w4
         log(f"config4: batched {c4['batched_Mops']} vs plain {c4['plain_per_batch_Mops']} M ops/s")
         levels = [float(q) for q in args.extras_qps.split(",")]
-        c5 = served_vs_plain(args.served_keys, levels, args.extras_requests, args.step_ops, dev)
+        c5 = served_vs_plain(args.served_keys, levels, args.extras_requests, args.step_ops, dev, args.served_mode)
         result["breakdown"]["batched_vs_plain"] = c4
         result["breakdown"]["served_vs_plain"] = {
             "keys": c5["keys"], "requests_per_level": c5["requests_per_level"],
@@ -506,7 +506,7 @@ def gpu_unique_keys(n: int, extra: int, seed: int, dev):
     return u32[:n].contiguous(), u32[n:].contiguous()
 
 
-def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev):
+def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev, mode: int = 2):
     """Served form (config 5): n-key tree resident in HBM; open-loop Poisson
     arrivals at each QPS level; request sizes log-uniform in [1, 16384] ops,
     80% find / 10% range (~32 keys) / 10% insert, one snapshot per request;
@@ -524,9 +524,7 @@ def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev):
     vals = init ^ torch.tensor(wl.VALUE_XOR - 2**32, dtype=torch.int32, device=dev)
     pool = int(n / 6) + (8 << 20)
     tree = mv.GpuVersionedBTree(pool_nodes=pool)
-    # Requests are small (mean about 1.7K ops), so runs would be one op anyway:
-    # the one-op worker has the smaller register footprint.
-    tree.set_worker_mode(0)
+    tree.set_worker_mode(mode)
     torch.cuda.synchronize()
     tb = time.perf_counter()
     tree.insert(init, vals)
@@ -581,7 +579,8 @@ def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev):
 def run_config5(args):
     import torch
     levels = [float(q) for q in args.qps.split(",")]
-    return served_vs_plain(args.served_keys, levels, args.requests, args.step_ops, torch.device("cuda:0"))
+    return served_vs_plain(args.served_keys, levels, args.requests, args.step_ops, torch.device("cuda:0"),
+                           args.served_mode)
 
 
 def cpu_model() -> str:
@@ -644,7 +643,8 @@ def main():
     ap.add_argument("--batches", type=int, default=4096, help="config 4: number of batches")
     ap.add_argument("--tune-batches", type=int, default=192,
                     help="config 4: batches in the timed tuning group (and in the warm-up group)")
-    ap.add_argument("--modes", default="0,1", help="config 4: worker modes to tune (0 one op per tile, 1 runs)")
+    ap.add_argument("--modes", default="1,2",
+                    help="config 4: worker modes to tune (0 one op per tile, 1 runs, 2 sorted)")
     ap.add_argument("--widths", default="16,32,64", help="config 4: node widths to tune")
     ap.add_argument("--waves", default="8,16,24,32", help="config 4: waves/CU to tune")
     ap.add_argument("--no-extras", action="store_true",
@@ -652,7 +652,9 @@ def main():
     ap.add_argument("--extras-qps", default="8000,32000", help="default run: config-5 QPS levels")
     ap.add_argument("--extras-requests", type=int, default=1000, help="default run: config-5 requests per level")
     ap.add_argument("--c4-node-width", type=int, default=32, help="default run: config-4 node width")
-    ap.add_argument("--c4-waves", type=int, default=8, help="default run: config-4 worker waves per CU")
+    ap.add_argument("--c4-waves", type=int, default=16, help="default run: config-4 worker waves per CU")
+    ap.add_argument("--c4-mode", type=int, default=2, help="default run: config-4 worker mode")
+    ap.add_argument("--served-mode", type=int, default=2, help="config 5: worker mode of the served steps")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()

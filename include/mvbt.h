@@ -251,7 +251,9 @@ typedef struct mvbt_op {
 /* No single reference counterpart: it generalises the reference's mixed kernels
  * concurrent_insert_range / concurrent_find_erase (gpu_versioned_blink_tree.hpp:218-294,
  * btree_kernels.hpp:31-306) to any mix, deterministically. Runs num_batches batches in
- * one persistent work-queue launch. batch_offsets (device, num_batches + 1 entries) are
+ * one launch sequence (a persistent work-queue worker in worker modes 0 and 1; the
+ * sorted form in mode 2, the default: every write grouped by leaf and applied in batch
+ * order, then every read at its batch's snapshot). batch_offsets (device, num_batches + 1 entries) are
  * absolute indices into ops: batch b is ops[offsets[b] .. offsets[b+1]), results go to
  * the same indices (so `results` must cover index offsets[num_batches] - 1), and
  * offsets[num_batches] - offsets[0] must equal num_ops (so a launch can cover any slice
@@ -284,10 +286,13 @@ mvbt_status mvbt_memory_stats_async(mvbt_tree* tree, mvbt_memstats* out, void* s
 /* Worker occupancy of the batched form in waves per CU (autotuner knob). */
 mvbt_status mvbt_set_worker_waves(mvbt_tree* tree, uint32_t waves_per_cu);
 
-/* Work unit of the batched form's worker (autotuner knob): 0 = one operation
- * per tile; 1 = runs of consecutive operations per tile, sized from the batch
- * sizes and handled with the chunk routing and batched leaf writes of the
- * per-batch kernels (default). Same results either way. */
+/* Work unit of the batched form (autotuner knob): 0 = one operation per tile of a
+ * persistent worker that opens batch after batch; 1 = the same worker with runs of
+ * consecutive operations per tile, handled with the chunk routing and batched leaf
+ * writes of the per-batch kernels; 2 = the sorted form (default): the call's writes
+ * are routed to their leaves, sorted by leaf (stable radix sort), and applied leaf group
+ * by leaf group in batch order by the tile that owns the group, then every find and
+ * range runs at its batch's snapshot in one pass. Same results in every mode. */
 mvbt_status mvbt_set_worker_mode(mvbt_tree* tree, uint32_t mode);
 
 /* ---- Device-side cooperative API ------------------------------------------ */

@@ -420,5 +420,7 @@ class GpuVersionedBTree(_TreeBase):
         _check(load_library().mvbt_set_worker_waves(self._h, waves_per_cu), "set_worker_waves")
 
     def set_worker_mode(self, mode: int):
-        """0: one operation per tile; 1: runs of operations per tile (default)."""
+        """0: one operation per tile; 1: runs of operations per tile; 2: the
+        sorted form (default): writes grouped by leaf in batch order, then
+        every read at its batch's snapshot."""
         _check(load_library().mvbt_set_worker_mode(self._h, mode), "set_worker_mode")

@@ -1,8 +1,10 @@
 """Autotuner for the batched form.
 
 Sweeps node width (16, 32, 64 pairs; the tile width equals the node width,
-one pair per lane), worker mode (one operation per tile, or runs of
-operations per tile) and worker occupancy (waves per CU), times the batched
+one pair per lane), worker mode (one operation per tile, runs of operations
+per tile, or the sorted form: writes grouped by leaf, then every read at its
+snapshot) and worker occupancy (waves per CU: of the persistent worker, or of
+the sorted form's write pass), times the batched
 form on a sample of the target workload against a tree of the target size,
 and returns the fastest configuration. Results can be cached in a JSON file
 keyed by device name and workload shape.
@@ -35,7 +37,7 @@ def _device_name() -> str:
 
 def tune_batched(initial_keys, initial_values, groups, L: int, dev,
                  node_widths=(16, 32, 64), waves=(8, 16, 24, 32), cache: str | None = None,
-                 log=None, modes=(1,)):
+                 log=None, modes=(1, 2)):
     """Returns (best TuneResult, all results).
 
     `groups` is (warm, timed): two disjoint lists of batches cut from one

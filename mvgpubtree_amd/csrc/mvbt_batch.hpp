@@ -28,6 +28,8 @@
 // (btree_kernels.hpp:31-306), which take per-tile snapshots inside the
 // kernel and are therefore not deterministic.
 #pragma once
+#include "mvbt_kernels.hpp"
+#include "mvbt_sort.hpp"
 #include "mvbt_tree.hpp"
 
 namespace mvbt {
@@ -69,6 +71,21 @@ struct alignas(128) batch_job {
   // poll the replica of their wave id, so polling spreads over L2 channels.
   // Replicas only grow (atomic max), so no replica ever moves backwards.
   int32_t frontier[8 * 32];
+  // Sorted form (worker mode 2): library-owned scratch, one entry per
+  // operation unless stated.
+  uint32_t* w_rel;       // write operations (index - first_op), in operation order
+  uint32_t* w_par;       // parent hint of each write (indexed like w_rel)
+  uint32_t* sort_k[2];   // leaf hints (sort keys), two buffers
+  uint32_t* sort_v[2];   // positions in w_rel (sort payloads), two buffers
+  uint32_t* rd_find;     // find operations (index - first_op), any order
+  uint32_t* rd_range;    // range operations (index - first_op), any order
+  uint32_t* seg;         // kSortSegments per-segment write counts
+  uint32_t* hist;        // kRadix * kSortSegments histogram + kRadix digit totals
+  // counts of the current call (zeroed by batch_setup_kernel), one line
+  uint32_t n_writes;
+  uint32_t n_finds;
+  uint32_t n_ranges;
+  uint32_t pad3[29];
 };
 constexpr uint32_t kFrontierCopies = 8;
 
@@ -107,6 +124,9 @@ __global__ void batch_setup_kernel(batch_job* job, tree_state* st, batch_params 
   job->next_op = first;
   job->nops = ok ? last : first;
   job->bad = ok ? 0u : 1u;
+  job->n_writes = 0;
+  job->n_finds = 0;
+  job->n_ranges = 0;
   if (!ok) atomic_or32(&st->error_flags, kErrBadBatch);
   for (uint32_t c = 0; c < kFrontierCopies; ++c) job->frontier[c * 32] = -1;
 }
@@ -234,7 +254,7 @@ __global__ void __launch_bounds__(256) batch_worker_kernel(tree_ptrs tp, batch_j
   using node_t = node<NW, true>;
   tile<NW> t;
   const unsigned long long nops = job->nops;
-  const uint32_t nb = job->nb, L = job->L, clock0 = job->clock0;
+  const uint32_t L = job->L, clock0 = job->clock0;
   const op_record* ops = job->ops;
   uint32_t* results = job->results;
   uint64_t* pairs = job->pairs;
@@ -535,6 +555,303 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
     if (!pending) have = false;
   }
   tv.finish(ctx, t);
+}
+
+// ---- The sorted form (worker mode 2) ------------------------------------------
+// The whole stream is known when the call starts, so it need not run batch
+// after batch. Because every read names its snapshot and the tree keeps
+// versions, a read of batch b gives the same answer after every write of the
+// call as before the writes of batch b: it reads the version stamped <=
+// clock0 + b. So the call applies all its writes first and then answers all
+// its reads in one parallel pass. The writes only have to keep, per leaf,
+// the order of their batches (version chains are stamp-ordered); writes to
+// different leaves are independent. The pipeline:
+//   split  the writes, in operation order, into one list; finds and ranges
+//          into two more (any order);
+//   route  every write to its leaf (the tree is quiescent: no writer runs);
+//   sort   the writes by leaf, stably (mvbt_sort.hpp), so each leaf's
+//          writes are contiguous and in operation order, i.e. batch order;
+//   write  each tile owns the leaf groups that start in its window of the
+//          sorted list (a group that runs past the window stays with the
+//          tile that holds its start), and applies each group's writes
+//          batch by batch: one round writes, for every group, the writes of
+//          its earliest remaining batch through the batched leaf phase with
+//          that batch's stamp (write_chunk_at); keys it hands back (split,
+//          moved right) go one by one. No other tile writes a group's leaves
+//          or the leaves split from them, so no leaf lock is ever contended;
+//   read   finds and ranges, each at its batch's snapshot, as plain
+//          snapshot reads (no writer runs any more).
+// No frontier and no per-batch wait: a call costs a few bulk passes however
+// many batches it holds. Results are identical to the per-batch path.
+
+// Wave-aggregated append of `v` to `list` (one atomic per wave).
+MVBT_DEV void append_wave(bool p, uint32_t* counter, uint32_t* list, uint32_t v) {
+  const uint64_t m = __builtin_amdgcn_ballot_w64(p);
+  if (!m) return;
+  const uint32_t lane = __lane_id(), leader = (uint32_t)__builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane == leader) base = atomic_add32(counter, (uint32_t)__builtin_popcountll(m));
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
+  if (p) list[base + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1))] = v;
+}
+
+// 6a. Writes per segment of the operations (segments as in the sort).
+// A rejected call writes zero counts: the scan and the radix passes that
+// follow do not look at `bad`, and their sizes come from these counts.
+__global__ void __launch_bounds__(kSortBlock) batch_split_count_kernel(batch_job* job) {
+  __shared__ uint32_t part[kSortBlock / 64];
+  const uint32_t n = ld32_coherent(&job->bad) ? 0u : (uint32_t)(job->nops - job->first_op);
+  const op_record* ops = job->ops + job->first_op;
+  uint32_t a, b;
+  segment_of(n, blockIdx.x, gridDim.x, a, b);
+  uint32_t c = 0;
+  for (uint32_t i = a + threadIdx.x; i < b; i += kSortBlock) {
+    const uint32_t ty = ops[i].type;
+    c += (ty == kOpInsert || ty == kOpErase) ? 1u : 0u;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < kSortBlock / 64; ++w) tot += part[w];
+    job->seg[blockIdx.x] = tot;
+  }
+}
+
+// 6b (after radix_scan_kernel turned the counts into offsets and the total
+// into n_writes). The writes go to w_rel in operation order; finds and
+// ranges to their lists.
+__global__ void __launch_bounds__(kSortBlock) batch_split_kernel(batch_job* job) {
+  constexpr uint32_t kWaves = kSortBlock / 64;
+  __shared__ uint32_t wsum[kWaves];
+  if (ld32_coherent(&job->bad)) return;
+  const uint32_t n = (uint32_t)(job->nops - job->first_op);
+  const op_record* ops = job->ops + job->first_op;
+  uint32_t* w_rel = job->w_rel;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint64_t below = (1ull << lane) - 1;
+  uint32_t a, b;
+  segment_of(n, blockIdx.x, gridDim.x, a, b);
+  uint32_t run = job->seg[blockIdx.x];
+  for (uint32_t c = a; c < b; c += kSortBlock) {
+    const uint32_t i = c + tid;
+    const uint32_t ty = i < b ? ops[i].type : 0xFFu;
+    const bool isw = ty == kOpInsert || ty == kOpErase;
+    const uint64_t mw = __builtin_amdgcn_ballot_w64(isw);
+    if (lane == 0) wsum[w] = (uint32_t)__builtin_popcountll(mw);
+    append_wave(ty == kOpFind, &job->n_finds, job->rd_find, i);
+    append_wave(ty == kOpRange, &job->n_ranges, job->rd_range, i);
+    __syncthreads();
+    uint32_t pos = run + (uint32_t)__builtin_popcountll(mw & below), tot = 0;
+#pragma unroll
+    for (uint32_t v = 0; v < kWaves; ++v) {
+      if (v < w) pos += wsum[v];
+      tot += wsum[v];
+    }
+    if (isw) w_rel[pos] = i;
+    run += tot;
+    __syncthreads();
+  }
+}
+
+// 7. Routes every write to its leaf: sort key = leaf (cap_ for a key that is
+// not routed: reserved keys, which fail in the write pass), payload = the
+// write's position in w_rel; the parent goes to w_par.
+template <int NW>
+__global__ void __launch_bounds__(256) batch_route_kernel(tree_ptrs tp, batch_job* job) {
+  if (ld32_coherent(&job->bad)) return;
+  const tree_view<NW, true> tv(tp);
+  tile<NW> t;
+  chunking<NW> c;
+  __shared__ route_cache<NW, kRangeCacheBytes> rc;
+  tv.build_route_cache(rc);
+  const uint32_t n = job->n_writes;
+  const op_record* ops = job->ops + job->first_op;
+  const uint32_t* w_rel = job->w_rel;
+  uint32_t* hint = job->sort_k[0];
+  uint32_t* pos = job->sort_v[0];
+  uint32_t* w_par = job->w_par;
+  for (uint64_t base = c.first; base < n; base += c.stride) {
+    const uint64_t i = base + t.rank();
+    const bool act = i < n;
+    const uint32_t key = act ? ops[w_rel[i]].key : 0u;
+    const bool go = act && valid_key(key);
+    const uint64_t q = t.ballot(go);
+    uint32_t st0 = kNone, sp0 = kNone, leaf = kNone, par = kNone;
+    if (go) st0 = tv.route_lane(rc, key, sp0);
+    tv.template descend_chunk<false>(key, q, leaf, par, t, q, st0, sp0);
+    if (act) {
+      const bool routed = go && leaf < tv.cap_;
+      hint[i] = routed ? leaf : tv.cap_;
+      pos[i] = (uint32_t)i;
+      w_par[i] = routed ? par : kNone;
+    }
+  }
+}
+
+// 8. The write pass over the sorted writes (buffers sort_k/v[which]).
+template <int NW>
+__global__ void __launch_bounds__(256) batch_sorted_write_kernel(tree_ptrs tp, batch_job* job, uint32_t which) {
+  if (ld32_coherent(&job->bad)) return;
+  const tree_view<NW, true> tv(tp);
+  tile<NW> t;
+  chunking<NW> c;
+  const uint32_t r = t.rank();
+  const uint32_t n = job->n_writes;
+  const uint32_t* hint = job->sort_k[which];
+  const uint32_t* pos = job->sort_v[which];
+  const uint32_t* w_rel = job->w_rel;
+  const uint32_t* w_par = job->w_par;
+  const uint32_t* op_batch = job->op_batch;
+  const op_record* ops = job->ops + job->first_op;
+  uint32_t* results = job->results + job->first_op;
+  const uint32_t clock0 = job->clock0;
+  write_ctx ctx = tv.begin_writes(t);
+  ctx.exact = 1;
+  ctx.track = 0;  // stamps are fixed per batch: clock0 + b + 1
+  for (uint64_t p = c.first; p < n; p += c.stride) {
+    uint64_t i = p + r;
+    bool start = i < n && (i == 0 || hint[i] != hint[i - 1]);
+    const uint64_t starts = t.ballot(start);
+    if (!starts) continue;  // the window lies inside a group that started earlier
+    // sub-windows of NW positions from the first group start in the window to
+    // the first group start at or past its end
+    const uint64_t wend = p + NW;
+    for (uint64_t a = p + (uint64_t)lowest_bit(starts);; a += NW) {
+      i = a + r;
+      const bool in = i < n;
+      const uint32_t h = in ? hint[i] : kNone;
+      start = in && (i == 0 || h != hint[i - 1]);
+      uint64_t mine = t.ballot(in);
+      const uint64_t foreign = t.ballot(start && i >= wend);
+      bool last = a + NW >= n;
+      if (foreign) {
+        mine &= (1ull << lowest_bit(foreign)) - 1;
+        last = true;
+      }
+      if (!mine) break;
+      const bool act = (mine >> r) & 1;
+      const uint32_t pv = act ? pos[i] : 0u;
+      const uint32_t rel = act ? w_rel[pv] : 0u;
+      const uint32_t par = act ? w_par[pv] : kNone;
+      op_record op{};
+      if (act) op = ops[rel];
+      const uint32_t b = act ? op_batch[rel] : 0u;
+      const uint32_t cur = clock0 + b + 1;
+      const uint32_t leaf = (act && h < tv.cap_) ? h : kNone;
+      // lane r's group begins at glead (lane 0 or a group start of this sub-window)
+      const uint64_t gs = t.ballot(start) | 1ull;
+      const uint64_t upto = (r == 63) ? ~0ull : ((2ull << r) - 1);
+      const uint64_t from_lead = ~((1ull << highest_bit(gs & upto)) - 1);
+      uint32_t res = 0;
+      // rounds: every group writes its earliest remaining batch (a group's
+      // lanes are in batch order, so that is its lowest remaining lane's)
+      for (uint64_t R = mine; R;) {
+        const uint64_t ahead = R & from_lead;
+        const uint32_t front = ahead ? (uint32_t)lowest_bit(ahead) : r;
+        const uint32_t fb = t.shfl(b, front);
+        const bool sel = ((R >> r) & 1) && b == fb;
+        const uint64_t S = t.ballot(sel);
+        for (int w = 0; w < 2; ++w) {
+          uint64_t q = t.ballot(sel && op.type == (w == 0 ? kOpInsert : kOpErase));
+          if (!q) continue;
+          ctx.want = (uint32_t)__builtin_popcountll(q);
+          q = (w == 0) ? tv.template write_chunk_at<0>(op.key, op.arg, q, leaf, cur, res, ctx, t)
+                       : tv.template write_chunk_at<1>(op.key, 0u, q, leaf, cur, res, ctx, t);
+          while (q) {
+            const uint32_t j = (uint32_t)lowest_bit(q);
+            ctx.want = (uint32_t)__builtin_popcountll(q);
+            q &= q - 1;
+            ctx.cur = t.shfl(cur, j);
+            const uint32_t k = t.shfl(op.key, j), lf = t.shfl(leaf, j), pr = t.shfl(par, j);
+            const uint32_t v = (w == 0) ? tv.template write<0>(k, t.shfl(op.arg, j), ctx, t, lf, pr)
+                                        : tv.template write<1>(k, 0u, ctx, t, lf, pr);
+            if (r == j) res = v;
+          }
+        }
+        R &= ~S;
+      }
+      if (act) results[rel] = res;
+      if (last) break;
+    }
+  }
+  tv.finish(ctx, t);
+}
+
+// 9. Finds, each at its batch's snapshot (no writer runs).
+template <int NW>
+__global__ void __launch_bounds__(kFindBlock) batch_sorted_find_kernel(tree_ptrs tp, const batch_job* job) {
+  if (ld32_coherent(&job->bad)) return;
+  const tree_view<NW, true> tv(tp);
+  tile<NW> t;
+  chunking<NW> c;
+  __shared__ route_cache<NW, kFindCacheBytes> rc;
+  tv.build_route_cache(rc);
+  const uint32_t n = job->n_finds;
+  const uint32_t* list = job->rd_find;
+  const uint32_t* op_batch = job->op_batch;
+  const op_record* ops = job->ops + job->first_op;
+  uint32_t* results = job->results + job->first_op;
+  const uint32_t clock0 = job->clock0;
+  for (uint64_t base = c.first; base < n; base += c.stride) {
+    const uint64_t i = base + t.rank();
+    const bool act = i < n;
+    const uint32_t rel = act ? list[i] : 0u;
+    const uint32_t key = act ? ops[rel].key : 0u;
+    const uint32_t ts = act ? clock0 + op_batch[rel] : 0u;
+    uint32_t start = kNone, unused;
+    if (act && valid_key(key)) start = tv.route_lane(rc, key, unused);
+    uint32_t result = kEmpty;
+    for (uint64_t q = t.ballot(act); q;) {
+      const uint32_t j = (uint32_t)lowest_bit(q);
+      q &= q - 1;
+      const uint32_t v = tv.template find<false>(t.shfl(key, j), t.shfl(ts, j), t, t.shfl(start, j));
+      if (t.rank() == j) result = v;
+    }
+    if (act) results[rel] = result;
+  }
+}
+
+// 10. Ranges, each at its batch's snapshot: the pass-based reader with a
+// snapshot and an output slot per lane.
+template <int NW>
+__global__ void __launch_bounds__(256) batch_sorted_range_kernel(tree_ptrs tp, const batch_job* job) {
+  if (ld32_coherent(&job->bad)) return;
+  using rc_t = route_cache<NW, kRangeCacheBytes>;
+  const tree_view<NW, true> tv(tp);
+  tile<NW> t;
+  chunking<NW> c;
+  __shared__ rc_t rc;
+  tv.build_route_cache(rc);
+  const uint32_t n = job->n_ranges, L = job->L;
+  const uint32_t* list = job->rd_range;
+  const uint32_t* op_batch = job->op_batch;
+  const op_record* ops = job->ops + job->first_op;
+  uint32_t* results = job->results + job->first_op;
+  uint64_t* pairs = job->pairs;
+  const unsigned long long pair_slots = job->pair_slots;
+  const uint32_t clock0 = job->clock0;
+  bool overflow = false, bad_slot = false;
+  for (uint64_t base = c.first; base < n; base += c.stride) {
+    const uint64_t i = base + t.rank();
+    const bool act = i < n;
+    const uint32_t rel = act ? list[i] : 0u;
+    op_record op{};
+    if (act) op = ops[rel];
+    const uint32_t ts = act ? clock0 + op_batch[rel] : 0u;
+    uint64_t* out = nullptr;
+    if (act && pairs) {
+      if (op.slot < pair_slots) out = pairs + (size_t)op.slot * L;
+      else bad_slot = true;
+    }
+    const uint32_t cnt = tv.template read_chunk<true, rc_t, true>(op.key == 0 ? 1u : op.key, op.arg, ts, out, L,
+                                                                  t.ballot(act), t, &rc);
+    if (act) results[rel] = cnt;
+    overflow |= act && cnt > L;
+  }
+  if (t.any(overflow)) tv.flag(kErrRangeOverflow, t);
+  if (t.any(bad_slot)) tv.flag(kErrCorrupt, t);
 }
 
 }  // namespace mvbt

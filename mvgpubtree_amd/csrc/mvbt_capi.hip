@@ -36,6 +36,8 @@ mvbt_status fail(mvbt_status s, const std::string& msg) {
       return fail(MVBT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
   } while (0)
 
+mvbt_status hip_fail(hipError_t e) { return fail(MVBT_ERR_HIP, std::string("launch: ") + hipGetErrorString(e)); }
+
 constexpr int kBlock = 256;
 
 }  // namespace
@@ -100,8 +102,14 @@ struct mvbt_tree {
   uint64_t b_hash_cap = 0;  // entries (a power of two)
   uint32_t* b_op_batch = nullptr;
   uint64_t b_op_cap = 0;    // entries
+  // sorted form (worker mode 2): 8 arrays of b_op_cap entries and the sort's
+  // segment counts and histograms, in one allocation
+  uint32_t* b_sorted = nullptr;
+  uint64_t b_sorted_cap = 0;  // entries per array
   uint32_t worker_waves = 16;
-  uint32_t worker_mode = 1;  // 0: one operation per tile; 1: chunks (batch_chunk_worker_kernel)
+  // 0: one operation per tile; 1: chunks (batch_chunk_worker_kernel);
+  // 2: sorted (writes grouped by leaf, then every read at its snapshot)
+  uint32_t worker_mode = 2;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   hipGraphNode_t graph_setup = nullptr;
@@ -219,6 +227,20 @@ template <int NW>
 void* worker_fn() {
   return reinterpret_cast<void*>(&batch_worker_kernel<NW>);
 }
+// The sorted form's width-specific kernels: route, write, find, range.
+struct sorted_fns {
+  void* route;
+  void* write;
+  void* find;
+  void* range;
+};
+template <int NW>
+sorted_fns sorted_fn() {
+  return sorted_fns{reinterpret_cast<void*>(&batch_route_kernel<NW>),
+                    reinterpret_cast<void*>(&batch_sorted_write_kernel<NW>),
+                    reinterpret_cast<void*>(&batch_sorted_find_kernel<NW>),
+                    reinterpret_cast<void*>(&batch_sorted_range_kernel<NW>)};
+}
 template <int NW>
 void* chunk_worker_fn() {
   return reinterpret_cast<void*>(&batch_chunk_worker_kernel<NW>);
@@ -233,6 +255,7 @@ struct dispatch_table {
   mvbt_status (*erase)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, hipStream_t);
   void* (*worker)();
   void* (*chunk_worker)();
+  sorted_fns (*sorted)();
   mvbt_status (*bulk)(mvbt_tree*, const uint32_t*, const uint32_t*, const bulk_plan&, hipStream_t);
   uint32_t data_lanes;
   using mixed_fn = mvbt_status (*)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, const uint32_t*,
@@ -245,7 +268,8 @@ template <int NW, bool VER>
 dispatch_table make_table() {
   using I = impl<NW, VER>;
   return dispatch_table{&I::init, &I::find, &I::range, &I::template write<0>, &I::template write<1>,
-                        VER ? &worker_fn<NW> : nullptr, VER ? &chunk_worker_fn<NW> : nullptr, &I::bulk,
+                        VER ? &worker_fn<NW> : nullptr, VER ? &chunk_worker_fn<NW> : nullptr,
+                        VER ? &sorted_fn<NW> : nullptr, &I::bulk,
                         I::data_lanes(),
                         &I::template mixed<true>, &I::template mixed<false>};
 }
@@ -495,14 +519,45 @@ uint64_t conflict_entries(uint64_t n) {
   return e;
 }
 
+// Entries of the sorted form's allocation before its histograms.
+constexpr uint64_t kSortedArrays = 8;
+uint64_t sorted_scratch_words(uint64_t cap) {
+  return kSortedArrays * cap + kSortSegments + (uint64_t)kRadix * kSortSegments + kRadix;
+}
+
+
+void destroy_graph(mvbt_tree* t);
+
 mvbt_status ensure_batch_scratch(mvbt_tree* t, uint32_t nb, uint64_t nops) {
   if (!t->job) {
     MVBT_HIP(hipMalloc(&t->job, sizeof(batch_job)));
     MVBT_HIP(hipMemset(t->job, 0, sizeof(batch_job)));
   }
   const uint64_t want_hash = conflict_entries(nops);
-  if (nb <= t->b_capacity && want_hash <= t->b_hash_cap && nops <= t->b_op_cap) return MVBT_OK;
+  const bool sorted = t->worker_mode == 2;
+  if (nb <= t->b_capacity && want_hash <= t->b_hash_cap && nops <= t->b_op_cap &&
+      (!sorted || nops <= t->b_sorted_cap))
+    return MVBT_OK;
   MVBT_HIP(hipDeviceSynchronize());
+  if (sorted && nops > t->b_sorted_cap) {
+    uint64_t cap = 1024;
+    while (cap < nops) cap *= 2;
+    if (t->b_sorted) (void)hipFree(t->b_sorted);
+    t->b_sorted = nullptr;
+    t->b_sorted_cap = 0;
+    MVBT_HIP(hipMalloc(&t->b_sorted, sorted_scratch_words(cap) * sizeof(uint32_t)));
+    t->b_sorted_cap = cap;
+    uint32_t* a = t->b_sorted;
+    // w_rel, w_par, sort_k[0..1], sort_v[0..1], rd_find, rd_range, seg, hist
+    uint32_t* sp[10];
+    for (uint64_t k = 0; k < kSortedArrays; ++k) sp[k] = a + k * cap;
+    sp[8] = a + kSortedArrays * cap;
+    sp[9] = sp[8] + kSortSegments;
+    static_assert(offsetof(batch_job, hist) == offsetof(batch_job, w_rel) + 9 * sizeof(void*), "sorted scratch layout");
+    MVBT_HIP(hipMemcpy(&t->job->w_rel, sp, sizeof(sp), hipMemcpyHostToDevice));
+    destroy_graph(t);  // the sort kernels of the graph take these pointers
+  }
+  if (nb <= t->b_capacity && want_hash <= t->b_hash_cap && nops <= t->b_op_cap) return MVBT_OK;
   if (nb > t->b_capacity) {
     uint32_t cap = 1024;
     while (cap < nb) cap *= 2;
@@ -547,6 +602,81 @@ uint32_t worker_blocks(const mvbt_tree* t) {
   return (uint32_t)std::max<uint64_t>(b, 1);
 }
 
+uint32_t worker_blocks(const mvbt_tree* t);
+
+// The sorted form's launch sequence after setup/clear/count: enqueued on a
+// stream, or added to a graph, through `add(fn, grid, block, args)`.
+// Radix passes cover the bits of the largest sort key (the pool capacity).
+uint32_t sort_passes(const mvbt_tree* t) {
+  const uint32_t cap = (uint32_t)t->cfg.pool_nodes;
+  const uint32_t bits = 32 - (uint32_t)__builtin_clz(cap | 1u);
+  return (bits + 7) / 8;
+}
+
+template <class Add>
+mvbt_status sorted_sequence(mvbt_tree* t, Add&& add, batch_job* job, uint32_t* a, uint64_t cap) {
+  uint32_t* sk[2] = {a + 2 * cap, a + 3 * cap};
+  uint32_t* sv[2] = {a + 4 * cap, a + 5 * cap};
+  uint32_t* seg = a + kSortedArrays * cap;
+  uint32_t* hist = seg + kSortSegments;
+  uint32_t* totals = hist + (uint64_t)kRadix * kSortSegments;
+  uint32_t* n_writes = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(job) + offsetof(batch_job, n_writes));
+  uint32_t segs = kSortSegments;
+  const sorted_fns f = table_for(t->nw, true)->sorted();
+  tree_ptrs tp = t->ptrs();
+  const uint32_t grid = (uint32_t)t->num_cus * 8;
+  {
+    void* args[] = {&job};
+    if (hipError_t e = add(reinterpret_cast<void*>(&batch_split_count_kernel), kSortSegments, kSortBlock, args))
+      return hip_fail(e);
+  }
+  {
+    void* args[] = {&seg, &segs, &n_writes};
+    if (hipError_t e = add(reinterpret_cast<void*>(&radix_scan_kernel), 1, kSortBlock, args)) return hip_fail(e);
+  }
+  {
+    void* args[] = {&job};
+    if (hipError_t e = add(reinterpret_cast<void*>(&batch_split_kernel), kSortSegments, kSortBlock, args))
+      return hip_fail(e);
+  }
+  {
+    void* args[] = {&tp, &job};
+    if (hipError_t e = add(f.route, grid, kBlock, args)) return hip_fail(e);
+  }
+  const uint32_t passes = sort_passes(t);
+  uint32_t shifts[4] = {0, 8, 16, 24};
+  for (uint32_t p = 0; p < passes; ++p) {
+    uint32_t* kin = sk[p & 1];
+    uint32_t* vin = sv[p & 1];
+    uint32_t* kout = sk[(p + 1) & 1];
+    uint32_t* vout = sv[(p + 1) & 1];
+    void* hargs[] = {&kin, &n_writes, &shifts[p], &hist};
+    if (hipError_t e = add(reinterpret_cast<void*>(&radix_hist_kernel), kSortSegments, kSortBlock, hargs))
+      return hip_fail(e);
+    void* cargs[] = {&hist, &segs, &totals};
+    if (hipError_t e = add(reinterpret_cast<void*>(&radix_scan_kernel), kRadix, kSortBlock, cargs))
+      return hip_fail(e);
+    void* sargs[] = {&kin, &vin, &kout, &vout, &n_writes, &shifts[p], &hist, &totals};
+    if (hipError_t e = add(reinterpret_cast<void*>(&radix_scatter_kernel), kSortSegments, kSortBlock, sargs))
+      return hip_fail(e);
+  }
+  uint32_t which = passes & 1;  // the buffer the last pass wrote
+  {
+    // the write pass's occupancy is the tunable one (worker waves per CU)
+    void* args[] = {&tp, &job, &which};
+    if (hipError_t e = add(f.write, worker_blocks(t), kBlock, args)) return hip_fail(e);
+  }
+  {
+    void* args[] = {&tp, &job};
+    if (hipError_t e = add(f.find, (uint32_t)t->num_cus * 2, kFindBlock, args)) return hip_fail(e);
+  }
+  {
+    void* args[] = {&tp, &job};
+    if (hipError_t e = add(f.range, grid, kBlock, args)) return hip_fail(e);
+  }
+  return MVBT_OK;
+}
+
 void* worker_kernel(const mvbt_tree* t) {
   const dispatch_table* tb = table_for(t->nw, true);
   return t->worker_mode == 1 ? tb->chunk_worker() : tb->worker();
@@ -589,10 +719,14 @@ mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
   t->graph_setup = node;
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_clear_kernel), t->num_cus * 4, kBlock, job_args));
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_count_kernel), t->num_cus * 4, kBlock, count_args));
-  MVBT_HIP(add(reinterpret_cast<void*>(&batch_frontier_kernel), t->num_cus, kBlock, job_args));
-  tree_ptrs tp = t->ptrs();
-  void* worker_args[] = {&tp, &job};
-  MVBT_HIP(add(worker_kernel(t), worker_blocks(t), kBlock, worker_args));
+  if (t->worker_mode == 2) {
+    if (mvbt_status rc = sorted_sequence(t, add, t->job, t->b_sorted, t->b_sorted_cap)) return rc;
+  } else {
+    MVBT_HIP(add(reinterpret_cast<void*>(&batch_frontier_kernel), t->num_cus, kBlock, job_args));
+    tree_ptrs tp = t->ptrs();
+    void* worker_args[] = {&tp, &job};
+    MVBT_HIP(add(worker_kernel(t), worker_blocks(t), kBlock, worker_args));
+  }
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_finish_kernel), 1, 64, job_args));
   MVBT_HIP(hipGraphInstantiate(&t->graph_exec, t->graph, nullptr, nullptr, 0));
   return MVBT_OK;
@@ -671,7 +805,15 @@ mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
   }
   mvbt_status rc = table_for(t->nw, t->versioned)->init(t.get());
   if (rc == MVBT_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(MVBT_ERR_HIP, "init kernel failed");
+  // the ordering events every writer uses, made here once rather than on first use
+  if (rc == MVBT_OK && (hipEventCreateWithFlags(&t->clock_ev, hipEventDisableTiming) != hipSuccess ||
+                        hipEventCreateWithFlags(&t->maint_ev, hipEventDisableTiming) != hipSuccess ||
+                        hipEventCreateWithFlags(&t->excl_ev, hipEventDisableTiming) != hipSuccess))
+    rc = fail(MVBT_ERR_HIP, "event creation failed");
   if (rc != MVBT_OK) {
+    if (t->clock_ev) (void)hipEventDestroy(t->clock_ev);
+    if (t->maint_ev) (void)hipEventDestroy(t->maint_ev);
+    if (t->excl_ev) (void)hipEventDestroy(t->excl_ev);
     cleanup();
     return rc;
   }
@@ -700,6 +842,7 @@ mvbt_status mvbt_destroy(mvbt_tree* t) {
   if (t->b_advance) (void)hipFree(t->b_advance);
   if (t->b_hash) (void)hipFree(t->b_hash);
   if (t->b_op_batch) (void)hipFree(t->b_op_batch);
+  if (t->b_sorted) (void)hipFree(t->b_sorted);
   if (t->excl_ev) (void)hipEventDestroy(t->excl_ev);
   delete t;
   return MVBT_OK;
@@ -1018,12 +1161,19 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
     hipLaunchKernelGGL(batch_setup_kernel, dim3(1), dim3(64), 0, s, t->job, t->st, p);
     hipLaunchKernelGGL(batch_clear_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job);
     hipLaunchKernelGGL(batch_count_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job, t->st);
-    hipLaunchKernelGGL(batch_frontier_kernel, dim3(t->num_cus), dim3(kBlock), 0, s, t->job);
-    tree_ptrs tp = t->ptrs();
-    batch_job* job = t->job;
-    void* wargs[] = {&tp, &job};
-    MVBT_HIP(hipLaunchKernel(worker_kernel(t), dim3(worker_blocks(t)), dim3(kBlock),
-                             wargs, 0, s));
+    if (t->worker_mode == 2) {
+      auto launch = [s](void* fn, unsigned grid, unsigned block, void** args) -> hipError_t {
+        return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
+      };
+      if (mvbt_status rc = sorted_sequence(t, launch, t->job, t->b_sorted, t->b_sorted_cap)) return rc;
+    } else {
+      hipLaunchKernelGGL(batch_frontier_kernel, dim3(t->num_cus), dim3(kBlock), 0, s, t->job);
+      tree_ptrs tp = t->ptrs();
+      batch_job* job = t->job;
+      void* wargs[] = {&tp, &job};
+      MVBT_HIP(hipLaunchKernel(worker_kernel(t), dim3(worker_blocks(t)), dim3(kBlock),
+                               wargs, 0, s));
+    }
     hipLaunchKernelGGL(batch_finish_kernel, dim3(1), dim3(64), 0, s, t->job);
   }
   MVBT_HIP(hipGetLastError());
@@ -1058,7 +1208,8 @@ mvbt_status mvbt_set_worker_waves(mvbt_tree* t, uint32_t waves_per_cu) {
 
 mvbt_status mvbt_set_worker_mode(mvbt_tree* t, uint32_t mode) {
   if (mvbt_status rc = check_tree(t)) return rc;
-  if (mode > 1) return fail(MVBT_ERR_INVALID_ARG, "worker mode is 0 (one operation per tile) or 1 (chunks)");
+  if (mode > 2)
+    return fail(MVBT_ERR_INVALID_ARG, "worker mode is 0 (one operation per tile), 1 (chunks) or 2 (sorted)");
   std::lock_guard<std::mutex> lk(t->mu);
   if (t->worker_mode != mode) {
     device_guard g(t->device);

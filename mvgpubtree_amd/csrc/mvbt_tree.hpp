@@ -194,9 +194,12 @@ struct tree_view : tree_ptrs {
   // writer) is written in place under its newer stamp, so every version
   // chain stays ordered by stamp, and the overlap is flagged.
   MVBT_DEV bool needs_copy(uint64_t ver, const write_ctx& c, const tile_t& t) const {
+    return needs_copy_at(ver, c.cur, t);
+  }
+  MVBT_DEV bool needs_copy_at(uint64_t ver, uint32_t cur, const tile_t& t) const {
     const uint32_t s = node_t::stamp(ver);
-    if (s > c.cur) flag(kErrStampOrder, t);
-    return s < c.cur;
+    if (s > cur) flag(kErrStampOrder, t);
+    return s < cur;
   }
 
   // ---- allocation (roles of the reference's device_bump_allocator.hpp:45-54
@@ -542,8 +545,10 @@ struct tree_view : tree_ptrs {
   // at `ts` or kEmpty. RANGE = true: range [lo, hi) with lo >= 1; pairs of the
   // key in lane j go to out + j * cap (at most cap of them; out may be null)
   // and the true count is returned. Keys outside `todo` return kEmpty / 0.
+  // LANES: `ts` is lane j's own snapshot and `out` lane j's own output (the
+  // sorted batched form's reads, each at its batch's snapshot).
   static constexpr int kReadGroup = NW < 8 ? NW : 8;
-  template <bool RANGE, class RC = route_cache<NW, 1024>>
+  template <bool RANGE, class RC = route_cache<NW, 1024>, bool LANES = false>
   MVBT_DEV uint32_t read_chunk(uint32_t lo, uint32_t hi, uint32_t ts, uint64_t* out, uint32_t cap,
                                uint64_t todo, const tile_t& t, const RC* rc = nullptr) const {
     enum : uint32_t { kRoute = 0, kArrive = 1, kChain = 2 };
@@ -577,6 +582,7 @@ struct tree_view : tree_ptrs {
           n.p = p[j];
           const uint64_t m = n.meta(t);
           const uint32_t k = t.shfl(lo, lj);
+          const uint32_t tsj = LANES ? t.shfl(ts, lj) : ts;
           uint32_t ph = t.shfl(phase, lj);
           uint32_t next = kNone;
           bool visit = false, done = false;
@@ -590,7 +596,7 @@ struct tree_view : tree_ptrs {
             bool later = false;
             if constexpr (VER) {
               ver = n.version(t);
-              later = ts != kLatest && node_t::stamp(ver) > ts;
+              later = tsj != kLatest && node_t::stamp(ver) > tsj;
             }
             if (later) {
               next = node_t::older(ver);
@@ -608,7 +614,9 @@ struct tree_view : tree_ptrs {
             if constexpr (RANGE) {
               const uint32_t h = t.shfl(hi, lj);
               c = t.shfl(cnt, lj);
-              c += n.collect(k, h, out ? out + (size_t)lj * cap : nullptr, c, cap, t);
+              uint64_t* oj = LANES ? reinterpret_cast<uint64_t*>(t.shfl64(reinterpret_cast<uint64_t>(out), lj))
+                                   : (out ? out + (size_t)lj * cap : nullptr);
+              c += n.collect(k, h, oj, c, cap, t);
               if (h <= node_t::high_key(m) || node_t::sibling(m) == kNoSibling) {
                 done = true;
               } else {
@@ -1295,6 +1303,14 @@ struct tree_view : tree_ptrs {
   template <int OP>
   MVBT_DEV uint64_t write_chunk(uint32_t key, uint32_t value, uint64_t q, uint32_t leaf,
                                 uint32_t& result, write_ctx& c, const tile_t& t) const {
+    return write_chunk_at<OP>(key, value, q, leaf, c.cur, result, c, t);
+  }
+  // The same with a stamp per lane (`cur`, equal for the lanes of one leaf):
+  // the sorted batched form writes several batches' epochs in one chunk.
+  // A tile that tracks the clock (c.track) uses c.cur for every lane.
+  template <int OP>
+  MVBT_DEV uint64_t write_chunk_at(uint32_t key, uint32_t value, uint64_t q, uint32_t leaf, uint32_t cur,
+                                   uint32_t& result, write_ctx& c, const tile_t& t) const {
     if constexpr (NW > 32) {
       return q;  // 64-pair nodes: register budget; per-key path
     } else {
@@ -1318,6 +1334,7 @@ struct tree_view : tree_ptrs {
       for (uint64_t m = lead & ~locked; m; m &= m - 1)
         left |= todo & t.ballot(my_lead == (uint32_t)lowest_bit(m));
       if (locked) stamp_under_lock(c, t);  // one clock read covers every leaf locked above
+      if (c.track) cur = c.cur;
       uint64_t pending = locked, prev = 0;
       for (;;) {
         uint64_t got = 0;
@@ -1387,7 +1404,7 @@ struct tree_view : tree_ptrs {
             if (chg) changed |= 1u << i;
             node_t o;
             o.p = img[i];
-            if (VER && chg && needs_copy(o.version(t), c, t)) need_copy |= 1u << i;
+            if (VER && chg && needs_copy_at(o.version(t), t.shfl(cur, gl[i]), t)) need_copy |= 1u << i;
           }
         }
         // 4. version copies (new epoch), then their version words
@@ -1411,7 +1428,7 @@ struct tree_view : tree_ptrs {
           }
           if (need_copy) {
             order_release();
-            if (cp != kNone) st_coherent(at(leaf) + node_t::kVerLane, pack(c.cur, cp));
+            if (cp != kNone) st_coherent(at(leaf) + node_t::kVerLane, pack(cur, cp));
             order_release();
           }
         }
@@ -1434,8 +1451,8 @@ struct tree_view : tree_ptrs {
           // nothing from a disabled source lane, and the leader may be one)
           const uint32_t target = t.shfl(leaf, gl[i]);
           if constexpr (VER) {
-            const uint32_t cpi = t.shfl(cp, gl[i]);
-            if (((need_copy >> i) & 1) && r == node_t::kVerLane) n.p = pack(c.cur, cpi);
+            const uint32_t cpi = t.shfl(cp, gl[i]), curi = t.shfl(cur, gl[i]);
+            if (((need_copy >> i) & 1) && r == node_t::kVerLane) n.p = pack(curi, cpi);
           }
           st_coherent(at(target) + r, n.p);
         }

@@ -28,11 +28,12 @@ def build_tree(w, dev, nw=16):
     return tree
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("nw", [16, 32, 64])
 def test_batched_form_matches_oracle(dev, nw, use_graph, mode):
-    """mode 0: one operation per tile; mode 1: runs of operations per tile."""
+    """mode 0: one operation per tile; mode 1: runs of operations per tile;
+    mode 2: writes sorted by leaf, then every read at its snapshot."""
     w = small_config4(seed=40 + nw)
     tree = build_tree(w, dev, nw)
     tree.set_worker_mode(mode)
@@ -152,3 +153,86 @@ def test_autotuner_sweeps_and_caches(dev, tmp_path):
     again, _ = autotune.tune_batched(w.initial_keys, w.initial_values, groups, w.range_capacity, dev,
                                      node_widths=(16, 32), waves=waves, modes=modes, cache=cache)
     assert again == best
+
+
+def hot_leaf_batches(seed, n_batches, hot):
+    """Batches whose writes all fall on a few hot leaves: every leaf's writes
+    span hundreds of batches (the sorted form's leaf groups then run past a
+    tile's window many times over and split repeatedly), values change on
+    every re-insert, and erased keys come back."""
+    rng = np.random.default_rng(seed)
+    hot_keys = wl.unique_keys(rng, hot)
+    hs = np.sort(hot_keys)
+    span = int(wl.span_for(48, hot))
+    batches = []
+    for _ in range(n_batches):
+        w = rng.choice(hot_keys, size=int(rng.integers(0, 48)), replace=False)
+        k = int(rng.integers(0, w.size + 1))
+        ik, ek = w[:k], w[k:]
+        iv = rng.integers(1, 0xFFFFFFFE, size=ik.size, dtype=np.uint64).astype(np.uint32)
+        fk = rng.choice(hot_keys, size=int(rng.integers(0, 12)))
+        lo = hs[rng.integers(0, hs.size, size=int(rng.integers(0, 6)))]
+        hi = np.minimum(lo.astype(np.uint64) + span, 0xFFFFFFFF).astype(np.uint32)
+        batches.append(wl.Batch(insert_keys=ik, insert_values=iv, erase_keys=ek, find_keys=fk, range_lo=lo,
+                                range_hi=hi))
+    return hot_keys, batches
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("nw", [16, 32, 64])
+def test_sorted_form_hot_leaves(dev, nw, use_graph):
+    """Many batches writing the same few leaves, through the sorted form:
+    per leaf, the writes of each batch are applied in batch order (each read
+    sees exactly the earlier batches), against the oracle."""
+    hot_keys, batches = hot_leaf_batches(91 + nw, 700, 3000)
+    init = np.sort(hot_keys[:1500])
+    tree = mv.GpuVersionedBTree(pool_nodes=1 << 18, node_width=nw)
+    tree.insert(to_dev(init, dev), to_dev(wl.value_of(init), dev))
+    tree.set_worker_mode(2)
+    L = 256
+    r = drivers.BatchedRunner(batches, L, dev, seed=nw)
+    res = to_np(r.run(tree, use_graph=use_graph))
+    want, wpairs, final = oracle.batch_expected(init, wl.value_of(init), r.ops_host, r.offsets_host, L, r.slots)
+    bad = np.nonzero(res != want)[0]
+    assert bad.size == 0, (bad[:10], r.ops_host[bad[:10]], res[bad[:10]], want[bad[:10]])
+    assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * L], wpairs)
+    exp = tree.export()
+    assert np.array_equal(exp, final.keys.astype(np.uint64) | (final.values.astype(np.uint64) << np.uint64(32)))
+    assert tree.error_flags() == 0
+    nk, _ = tree.validate_tree_structure()
+    assert nk == final.keys.size
+    tree.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_batched_read_only_and_write_only_calls(dev, mode):
+    """A call with no writes, then one with no reads, then one with reserved
+    keys among its writes and reads (the writes fail with 0xFFFFFFFF and the
+    invalid-key flag; reads of them miss)."""
+    w = small_config4(seed=300 + mode, num_batches=30)
+    tree = build_tree(w, dev, 32)
+    tree.set_worker_mode(mode)
+    L = w.range_capacity
+    ks = np.sort(w.initial_keys)
+    vs = wl.value_of(ks)
+    ro = [wl.Batch(find_keys=b.find_keys, range_lo=b.range_lo, range_hi=b.range_hi) for b in w.batches]
+    wo = [wl.Batch(insert_keys=b.insert_keys, insert_values=b.insert_values, erase_keys=b.erase_keys)
+          for b in w.batches]
+    for batches in (ro, wo):
+        r = drivers.BatchedRunner(batches, L, dev, seed=mode)
+        res = to_np(r.run(tree, use_graph=True))
+        want, wpairs, final = oracle.batch_expected(ks, vs, r.ops_host, r.offsets_host, L, r.slots)
+        assert np.array_equal(res, want)
+        assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * L], wpairs)
+        ks, vs = final.keys, final.values
+    assert tree.error_flags() == 0
+    bad = [wl.Batch(insert_keys=np.array([0, 5], np.uint32), insert_values=np.array([1, 2], np.uint32),
+                    erase_keys=np.array([0xFFFFFFFF], np.uint32), find_keys=np.array([0, 5, 0xFFFFFFFF], np.uint32)),
+           wl.Batch(find_keys=np.array([5], np.uint32), range_lo=np.array([0], np.uint32),
+                    range_hi=np.array([0xFFFFFFFF], np.uint32))]
+    r = drivers.BatchedRunner(bad, 1 << 20, dev, seed=0)
+    res = to_np(r.run(tree, use_graph=False))
+    want, _, final = oracle.batch_expected(ks, vs, r.ops_host, r.offsets_host, 1 << 20, r.slots)
+    assert np.array_equal(res, want)
+    assert tree.error_flags() == 4  # MVBT_FLAG_INVALID_KEY
+    tree.close()

@@ -180,7 +180,8 @@ def test_batched_call_is_ordered_against_other_streams(dev):
     """insert X on stream B, run_batches on stream A, insert Y on stream B:
     the batches see X and not Y, whatever the device interleaving."""
     import torch
-    w = wl.config4(seed=62, n_initial=200_000, num_batches=64, min_batch=32, max_batch=8192, range_capacity=256)
+    # ranges sized for the initial keys read a tree that X doubles: capacity 512
+    w = wl.config4(seed=62, n_initial=200_000, num_batches=64, min_batch=32, max_batch=8192, range_capacity=512)
     rng = np.random.default_rng(5)
     live = np.sort(np.concatenate([w.initial_keys] + [b.insert_keys for b in w.batches]))
     xy = wl.unique_keys(rng, 400_000, exclude_sorted=live)
@@ -191,7 +192,7 @@ def test_batched_call_is_ordered_against_other_streams(dev):
         b.find_keys = np.concatenate([b.find_keys, extra])
     tree = mv.GpuVersionedBTree(pool_nodes=1 << 19)
     tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
-    r = drivers.BatchedRunner(w.batches, 256, dev, seed=3)
+    r = drivers.BatchedRunner(w.batches, 512, dev, seed=3)
     dx, dxv = to_dev(x, dev), to_dev(wl.value_of(x), dev)
     dy, dyv = to_dev(y, dev), to_dev(wl.value_of(y), dev)
     sync()
@@ -201,9 +202,9 @@ def test_batched_call_is_ordered_against_other_streams(dev):
     tree.insert(dy, dyv, stream=sb)
     sync()
     k0 = np.concatenate([w.initial_keys, x])
-    want, wpairs, m = oracle.replay_batches(k0, wl.value_of(k0), r.ops_host, r.offsets_host, 256, r.slots)
+    want, wpairs, m = oracle.replay_batches(k0, wl.value_of(k0), r.ops_host, r.offsets_host, 512, r.slots)
     assert np.array_equal(to_np(r.results)[: r.nops], want)
-    assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * 256], wpairs)
+    assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * 512], wpairs)
     m.insert(y, wl.value_of(y))
     assert np.array_equal(tree.export(), m.export())
     assert tree.error_flags() == 0

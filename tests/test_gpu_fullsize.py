@@ -129,8 +129,7 @@ def test_config4_full_size(dev):
     nw = 32
     tree = mv.GpuVersionedBTree(pool_nodes=int(40_000_000 / 4 * (16 / nw) + 2 * n_ops) + (1 << 20), node_width=nw)
     tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
-    tree.set_worker_mode(1)
-    tree.set_worker_waves(8)
+    tree.set_worker_mode(2)
     r = drivers.BatchedRunner(w.batches, L, dev)
     r.run(tree, use_graph=True)
     sync()
@@ -173,7 +172,7 @@ def test_config5_full_size(dev):
     ks = ks_dev.cpu().numpy().view(np.uint32)
     note(f"config5: {n} keys generated ({time.time() - t0:.1f}s)")
     tree = mv.GpuVersionedBTree(pool_nodes=int(n / 6) + (8 << 20))
-    tree.set_worker_mode(0)
+    tree.set_worker_mode(2)
     order = torch.randperm(n, generator=g, device=dev)
     k_ins = ks_dev[order]
     del order
