@@ -378,11 +378,12 @@ def run_config3(args):
             "reclaimed_MB": round(st.reclaimed_nodes * st.node_bytes / 1e6, 1), "timeline": timeline}
 
 
-def batched_vs_plain(w, L, dev, nw: int, mode: int, waves: int):
-    """Config 4 on a fresh tree per path: the whole stream in one persistent
-    launch through the captured hipGraph, against the plain per-batch launch
+def batched_vs_plain(w, L, dev, nw: int, mode: int, waves: int, plain: bool = True):
+    """Config 4 on a fresh tree per path: the whole stream in one batched
+    call through the captured hipGraph, against the plain per-batch launch
     path (C++ loop over the C ABI: snapshot, find, range, insert, erase,
-    release per batch). Both start from the same 40M-key tree."""
+    release per batch). Both start from the same 40M-key tree. plain=False
+    runs the batched form only (profiling)."""
     import torch
 
     import mvgpubtree_amd as mv
@@ -426,23 +427,26 @@ def batched_vs_plain(w, L, dev, nw: int, mode: int, waves: int):
     tree.close()
     del br
 
-    pr = drivers.PerBatchRunner(w.batches, L, dev)
-    tree = fresh_tree()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    pr.run(tree)
-    torch.cuda.synchronize()
-    plain_s = time.perf_counter() - t0
-    flags |= tree.error_flags()
-    tree.close()
-    del pr, dk, dv
+    plain_s = float("nan")
+    if plain:
+        pr = drivers.PerBatchRunner(w.batches, L, dev)
+        tree = fresh_tree()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pr.run(tree)
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t0
+        flags |= tree.error_flags()
+        tree.close()
+        del pr
+    del dk, dv
     if flags:
         raise RuntimeError(f"device error flags 0x{flags:x}")
     return {
         "batched_Mops": round(n_ops / batched_s / 1e6, 2),
-        "plain_per_batch_Mops": round(n_ops / plain_s / 1e6, 2),
-        "speedup": round(plain_s / batched_s, 2),
-        "batched_ms": round(batched_s * 1e3, 2), "plain_ms": round(plain_s * 1e3, 2),
+        "plain_per_batch_Mops": round(n_ops / plain_s / 1e6, 2) if plain else None,
+        "speedup": round(plain_s / batched_s, 2) if plain else None,
+        "batched_ms": round(batched_s * 1e3, 2), "plain_ms": round(plain_s * 1e3, 2) if plain else None,
         "ops": n_ops, "batches": len(w.batches), "initial_keys": int(n0),
         "failed_writes": failed,
         "node_width": nw, "worker_mode": mode, "waves_per_cu": waves,
@@ -480,7 +484,7 @@ def run_config4(args):
         best, allr = autotune.TuneResult(widths[0], waves[0], 0.0, 0.0, modes[0]), []
     log(f"autotune best: node width {best.node_width}, worker mode {best.worker_mode}, "
         f"{best.waves_per_cu} waves/CU")
-    out = batched_vs_plain(w, L, dev, best.node_width, best.worker_mode, best.waves_per_cu)
+    out = batched_vs_plain(w, L, dev, best.node_width, best.worker_mode, best.waves_per_cu, plain=not args.no_plain)
     out["tuning"] = [{"nw": r.node_width, "mode": r.worker_mode, "waves": r.waves_per_cu, "Mops": round(r.mops, 1)}
                      for r in allr]
     return out
@@ -646,6 +650,7 @@ def main():
     ap.add_argument("--modes", default="1,2",
                     help="config 4: worker modes to tune (0 one op per tile, 1 runs, 2 sorted)")
     ap.add_argument("--widths", default="16,32,64", help="config 4: node widths to tune")
+    ap.add_argument("--no-plain", action="store_true", help="config 4: batched form only (profiling)")
     ap.add_argument("--waves", default="8,16,24,32", help="config 4: waves/CU to tune")
     ap.add_argument("--no-extras", action="store_true",
                     help="default run: skip the config-4 and config-5 legs of the breakdown")
