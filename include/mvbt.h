@@ -278,6 +278,25 @@ mvbt_status mvbt_run_batches(mvbt_tree* tree, const mvbt_op* ops, uint64_t num_o
                              uint32_t* results, mvbt_pair* pairs, uint64_t pair_slots, uint32_t L,
                              uint32_t* snapshot_base, void* stream, uint32_t flags);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * mvbt_run_batches in the sorted form (worker mode 2) with every scratch array in one
+ * static device array of the library (about 24 MB on each device it runs on, held
+ * until the process ends) instead of handle-owned allocations. Same arguments,
+ * results and rejections as mvbt_run_batches (no flags: launched directly, no graph).
+ * Limits: num_ops <= 524288 (2^19) and num_batches <= 16384 (2^14) per call, else
+ * MVBT_ERR_INVALID_ARG and nothing is launched. Calls no hipMalloc/hipFree (the
+ * first write on a stream creates that stream's ordering event, a host-side object).
+ * Every call on one device shares that device's scratch: calls that overlap on one
+ * device (two streams, or two handles on the same device) race and must be ordered
+ * by the caller. */
+mvbt_status mvbt_run_batches_noalloc(mvbt_tree* tree, const mvbt_op* ops, uint64_t num_ops,
+                                     const uint64_t* batch_offsets, uint32_t num_batches,
+                                     uint32_t* results, mvbt_pair* pairs, uint64_t pair_slots,
+                                     uint32_t L, uint32_t* snapshot_base, void* stream);
+#define MVBT_NOALLOC_MAX_OPS (1u << 19)
+#define MVBT_NOALLOC_MAX_BATCHES (1u << 14)
+
 /* Stream-ordered memory accounting: computes the mvbt_memstats of the point in
  * `stream` where it is enqueued and stores it to `out` (device or pinned host
  * memory) without synchronising the host. No reference counterpart. */

@@ -43,6 +43,7 @@ FLAG_SNAPSHOT_ID = 0x40
 FLAG_STAMP_ORDER = 0x80
 FLAG_BAD_BATCH = 0x100
 PLAIN, VERSIONED = 0, 1
+NOALLOC_MAX_OPS, NOALLOC_MAX_BATCHES = 1 << 19, 1 << 14  # mvbt_run_batches_noalloc limits
 
 _STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "UNSUPPORTED", 5: "DEVICE_FLAGS"}
 
@@ -53,7 +54,7 @@ EXPORTED_SYMBOLS = (
     "mvbt_memory_stats", "mvbt_error_flags", "mvbt_clear_error_flags", "mvbt_validate",
     "mvbt_export", "mvbt_node_width", "mvbt_kind_of", "mvbt_run_batches", "mvbt_set_worker_waves",
     "mvbt_memory_stats_async", "mvbt_device_view", "mvbt_bulk_build", "mvbt_concurrent_insert_range",
-    "mvbt_concurrent_find_erase", "mvbt_export_dot", "mvbt_set_worker_mode",
+    "mvbt_concurrent_find_erase", "mvbt_export_dot", "mvbt_set_worker_mode", "mvbt_run_batches_noalloc",
 )
 
 
@@ -136,6 +137,7 @@ def load_library():
         "mvbt_node_width": ([vp, u32p], i32),
         "mvbt_kind_of": ([vp, u32p], i32),
         "mvbt_run_batches": ([vp, vp, u64, vp, u32, vp, vp, u64, u32, vp, vp, u32], i32),
+        "mvbt_run_batches_noalloc": ([vp, vp, u64, vp, u32, vp, vp, u64, u32, vp, vp], i32),
         "mvbt_set_worker_waves": ([vp, u32], i32),
         "mvbt_set_worker_mode": ([vp, u32], i32),
         "mvbt_memory_stats_async": ([vp, vp, vp], i32),
@@ -396,19 +398,28 @@ class GpuVersionedBTree(_TreeBase):
 
     def run_batches(self, ops, batch_offsets, results=None, pairs=None, pair_slots: int = 0,
                     average_range_length: int = 0, snapshot_base=None, stream=None,
-                    use_graph: bool = False, num_ops: int | None = None):
+                    use_graph: bool = False, num_ops: int | None = None, no_alloc: bool = False):
         """Batched form: `ops` is an int32 device tensor of shape (n, 4) holding
         {type, key, arg, slot} (OP_* types); `batch_offsets` an int64 device
         tensor of num_batches + 1 absolute op indices into `ops`; results go to
         the same indices (the default results tensor has one entry per row of
         `ops`). `num_ops` = offsets[-1] - offsets[0] (default: all of `ops`).
         A batch must not write one key twice (the call is then rejected:
-        FLAG_BAD_BATCH). Returns the results tensor."""
+        FLAG_BAD_BATCH). no_alloc=True goes through mvbt_run_batches_noalloc
+        (static scratch, at most NOALLOC_MAX_OPS operations and
+        NOALLOC_MAX_BATCHES batches; pass `results` to allocate nothing here
+        either). Returns the results tensor."""
         torch = _torch()
         n = ops.shape[0] if num_ops is None else num_ops
         if results is None:
             results = torch.empty(ops.shape[0], dtype=torch.int32, device=ops.device)
         nb = batch_offsets.numel() - 1
+        if no_alloc:
+            _check(load_library().mvbt_run_batches_noalloc(
+                self._h, _ptr(ops, 4, "ops"), n, _ptr(batch_offsets, 8, "batch_offsets"), nb,
+                _ptr(results, 4, "results"), _ptr(pairs, 8, "pairs"), pair_slots,
+                average_range_length, _ptr(snapshot_base, 4, "snapshot_base"), _stream(stream)), "run_batches_noalloc")
+            return results
         _check(load_library().mvbt_run_batches(
             self._h, _ptr(ops, 4, "ops"), n, _ptr(batch_offsets, 8, "batch_offsets"), nb,
             _ptr(results, 4, "results"), _ptr(pairs, 8, "pairs"), pair_slots,

@@ -131,6 +131,36 @@ __global__ void batch_setup_kernel(batch_job* job, tree_state* st, batch_params 
   for (uint32_t c = 0; c < kFrontierCopies; ++c) job->frontier[c * 32] = -1;
 }
 
+// 0. Points a job descriptor at its scratch arrays from the device side (the
+// allocation-free entry point: the job and its scratch all lie in one static
+// device array, so no host copy is needed).
+struct scratch_layout {
+  uint32_t* nwrites;
+  uint32_t* done;
+  int32_t* advance;
+  uint32_t* hash;
+  uint32_t* op_batch;
+  uint32_t* sorted[10];  // w_rel, w_par, sort_k[0..1], sort_v[0..1], rd_find, rd_range, seg, hist
+};
+__global__ void batch_bind_kernel(batch_job* job, scratch_layout l) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  job->nwrites = l.nwrites;
+  job->done = l.done;
+  job->advance = l.advance;
+  job->hash = l.hash;
+  job->op_batch = l.op_batch;
+  job->w_rel = l.sorted[0];
+  job->w_par = l.sorted[1];
+  job->sort_k[0] = l.sorted[2];
+  job->sort_k[1] = l.sorted[3];
+  job->sort_v[0] = l.sorted[4];
+  job->sort_v[1] = l.sorted[5];
+  job->rd_find = l.sorted[6];
+  job->rd_range = l.sorted[7];
+  job->seg = l.sorted[8];
+  job->hist = l.sorted[9];
+}
+
 // 1'. Clears the conflict table (a fill kernel, so the launch graph holds no
 // memset node).
 __global__ void batch_clear_kernel(batch_job* job) {

@@ -513,7 +513,7 @@ mvbt_status walk_tree(const host_walk& w, std::vector<mvbt_pair>* pairs, uint64_
 
 // Conflict-table entries for a call of n operations: a power of two at least
 // twice n (and at least 1024).
-uint64_t conflict_entries(uint64_t n) {
+constexpr uint64_t conflict_entries(uint64_t n) {
   uint64_t e = 1024;
   while (e < 2 * n) e *= 2;
   return e;
@@ -521,10 +521,53 @@ uint64_t conflict_entries(uint64_t n) {
 
 // Entries of the sorted form's allocation before its histograms.
 constexpr uint64_t kSortedArrays = 8;
-uint64_t sorted_scratch_words(uint64_t cap) {
+constexpr uint64_t sorted_scratch_words(uint64_t cap) {
   return kSortedArrays * cap + kSortSegments + (uint64_t)kRadix * kSortSegments + kRadix;
 }
 
+// ---- allocation-free batched form (mvbt_run_batches_noalloc) ----------------
+// Limits of one call, and the static scratch they need: the job descriptor,
+// three per-batch counters, the conflict table, each op's batch, and the
+// sorted form's arrays (about 24 MB per device).
+constexpr uint64_t kNoAllocMaxOps = 1ull << 19;
+constexpr uint32_t kNoAllocMaxBatches = 1u << 14;
+constexpr uint64_t kNoAllocHash = conflict_entries(kNoAllocMaxOps);
+constexpr uint64_t kNoAllocJobWords = (sizeof(batch_job) + 255) / 256 * 64;
+constexpr uint64_t kNoAllocWords = kNoAllocJobWords + 3ull * kNoAllocMaxBatches + kNoAllocHash + kNoAllocMaxOps +
+                                   sorted_scratch_words(kNoAllocMaxOps);
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+// Scratch of the allocation-free entry point. Each device the library runs on
+// holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device
+// (g_scratch_ptr), never through this name.
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+namespace {
+
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices];  // per device: the address of `scratch` there
+
+// Looks up the device address of `scratch` on `device` (which must be
+// current) once per device.
+mvbt_status bind_noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(MVBT_ERR_UNSUPPORTED, "device ordinal above 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return MVBT_OK;
+  void* p = nullptr;
+  MVBT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(MVBT_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+  g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+  return MVBT_OK;
+}
+
+uint32_t* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 void destroy_graph(mvbt_tree* t);
 
@@ -769,6 +812,9 @@ mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
   hipDeviceProp_t prop;
   MVBT_HIP(hipGetDeviceProperties(&prop, dev));
   t->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  // The allocation-free entry point's scratch address on this device (looked
+  // up again by that entry point if this fails).
+  if (t->versioned && bind_noalloc_scratch(dev) != MVBT_OK) g_last_error.clear();
 
   auto cleanup = [&]() {
     if (t->nodes) (void)hipFree(t->nodes);
@@ -1176,6 +1222,67 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
     }
     hipLaunchKernelGGL(batch_finish_kernel, dim3(1), dim3(64), 0, s, t->job);
   }
+  MVBT_HIP(hipGetLastError());
+  if (mvbt_status rc = clock_advanced(t, s, num_batches)) return rc;
+  if (mvbt_status rc = isolated_done(t, s)) return rc;
+  return after_write(t, s);
+}
+
+mvbt_status mvbt_run_batches_noalloc(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
+                                     const uint64_t* batch_offsets, uint32_t num_batches, uint32_t* results,
+                                     mvbt_pair* pairs, uint64_t pair_slots, uint32_t L, uint32_t* snapshot_base,
+                                     void* stream) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!t->versioned) return fail(MVBT_ERR_UNSUPPORTED, "the batched form needs a versioned tree");
+  if (num_ops == 0 || num_batches == 0) return MVBT_OK;
+  if (!ops || !batch_offsets || !results) return fail(MVBT_ERR_INVALID_ARG, "null ops, offsets or results");
+  if (num_ops > kNoAllocMaxOps || num_batches > kNoAllocMaxBatches)
+    return fail(MVBT_ERR_INVALID_ARG, "mvbt_run_batches_noalloc takes at most " + std::to_string(kNoAllocMaxOps) +
+                                          " operations and " + std::to_string(kNoAllocMaxBatches) +
+                                          " batches per call (got " + std::to_string(num_ops) + " and " +
+                                          std::to_string(num_batches) + ")");
+  if (pairs && L == 0) return fail(MVBT_ERR_INVALID_ARG, "L must be > 0 when pairs is given");
+  device_guard g(t->device);
+  std::lock_guard<std::mutex> lk(t->mu);
+  if (mvbt_status rc = bind_noalloc_scratch(t->device)) return rc;
+  uint32_t* base = noalloc_scratch(t->device);
+  if (mvbt_status rc = reserve_ids(t, num_batches)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (mvbt_status rc = before_write(t, s)) return rc;
+  if (mvbt_status rc = isolate(t, s)) return rc;
+  // layout: job | nwrites, done, advance (per batch) | conflict table | op batch | sorted arrays
+  batch_job* job = reinterpret_cast<batch_job*>(base);
+  scratch_layout l{};
+  uint32_t* w = base + kNoAllocJobWords;
+  l.nwrites = w;
+  l.done = w + kNoAllocMaxBatches;
+  l.advance = reinterpret_cast<int32_t*>(w + 2 * kNoAllocMaxBatches);
+  l.hash = w + 3 * kNoAllocMaxBatches;
+  l.op_batch = l.hash + kNoAllocHash;
+  uint32_t* sorted = l.op_batch + kNoAllocMaxOps;
+  for (uint64_t k = 0; k < kSortedArrays; ++k) l.sorted[k] = sorted + k * kNoAllocMaxOps;
+  l.sorted[8] = sorted + kSortedArrays * kNoAllocMaxOps;
+  l.sorted[9] = l.sorted[8] + kSortSegments;
+  batch_params p{};
+  p.ops = reinterpret_cast<const op_record*>(ops);
+  p.nops = num_ops;
+  p.offsets = reinterpret_cast<const unsigned long long*>(batch_offsets);
+  p.nb = num_batches;
+  p.L = L;
+  p.results = results;
+  p.pairs = reinterpret_cast<uint64_t*>(pairs);
+  p.pair_slots = pairs ? pair_slots : 0;
+  p.base_out = snapshot_base;
+  p.hash_mask = (uint32_t)(conflict_entries(num_ops) - 1);
+  hipLaunchKernelGGL(batch_bind_kernel, dim3(1), dim3(64), 0, s, job, l);
+  hipLaunchKernelGGL(batch_setup_kernel, dim3(1), dim3(64), 0, s, job, t->st, p);
+  hipLaunchKernelGGL(batch_clear_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, job);
+  hipLaunchKernelGGL(batch_count_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, job, t->st);
+  auto launch = [s](void* fn, unsigned grid, unsigned block, void** args) -> hipError_t {
+    return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
+  };
+  if (mvbt_status rc = sorted_sequence(t, launch, job, sorted, kNoAllocMaxOps)) return rc;
+  hipLaunchKernelGGL(batch_finish_kernel, dim3(1), dim3(64), 0, s, job);
   MVBT_HIP(hipGetLastError());
   if (mvbt_status rc = clock_advanced(t, s, num_batches)) return rc;
   if (mvbt_status rc = isolated_done(t, s)) return rc;
