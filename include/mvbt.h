@@ -97,6 +97,21 @@ typedef struct mvbt_memstats {
   uint64_t retired_total;   /* total copies ever retired */
 } mvbt_memstats;
 
+/* One reclamation epoch of the memory timeline: the {usage, total allocated} pair the
+ * reference records at each epoch advance under COLLECT_RECLAIMER_STATS
+ * (memory_reclaimer.hpp:36-41,387-398), with the tree clock and the epoch's own figures. */
+typedef struct mvbt_epoch_stat {
+  uint64_t epoch;           /* reclamation number since the tree was created (0, 1, ...) */
+  uint64_t clock;           /* tree clock (the next snapshot id) at the reclamation */
+  uint64_t live_nodes;      /* slots in use after it (reference: usage) */
+  uint64_t allocated_total; /* slots ever handed out: live + reclaimed (reference: total allocated) */
+  uint64_t freed_nodes;     /* slots this reclamation returned */
+  uint64_t retired_pending; /* retired copies still awaiting reclamation after it */
+  uint64_t reclaimed_total; /* slots returned by every reclamation so far */
+  uint64_t capacity_nodes;  /* pool slots */
+} mvbt_epoch_stat;
+#define MVBT_SERIES_CAPACITY 4096u /* epochs kept on the device (the latest) */
+
 typedef struct mvbt_tree mvbt_tree;
 
 /* Human-readable text of the last error on the calling thread. */
@@ -164,6 +179,13 @@ mvbt_status mvbt_reclaim(mvbt_tree* tree, void* stream);
 /* Memory accounting (reference compute_memory_usage gpu_versioned_blink_tree.hpp:1442-1446,
  * COLLECT_ALLOCATOR_STATS slab_alloc.hpp:489-540). Synchronous. */
 mvbt_status mvbt_memory_stats(mvbt_tree* tree, mvbt_memstats* out);
+
+/* The memory timeline (versioned trees): every reclamation - mvbt_reclaim, and the
+ * reclamations a batched call makes between its windows - appends one mvbt_epoch_stat on
+ * the device, in a ring of the latest MVBT_SERIES_CAPACITY. Copies the latest
+ * min(recorded, capacity) of them, oldest first, to `out` (host memory) and their number
+ * to *count. Synchronous. Reference: compute_epochs_stats (memory_reclaimer.hpp:114-133). */
+mvbt_status mvbt_memory_series(mvbt_tree* tree, mvbt_epoch_stat* out, uint32_t capacity, uint32_t* count);
 
 /* Sticky device error flags (MVBT_FLAG_*). Synchronous. */
 mvbt_status mvbt_error_flags(mvbt_tree* tree, uint32_t* flags);
@@ -246,7 +268,8 @@ typedef struct mvbt_op {
   uint32_t slot;
 } mvbt_op;
 
-#define MVBT_USE_GRAPH 0x2u /* launch through the handle's captured hipGraph */
+#define MVBT_USE_GRAPH 0x2u  /* launch through the handle's captured hipGraph */
+#define MVBT_NO_RECLAIM 0x4u /* sorted form: keep every version copy until mvbt_reclaim */
 
 /* No single reference counterpart: it generalises the reference's mixed kernels
  * concurrent_insert_range / concurrent_find_erase (gpu_versioned_blink_tree.hpp:218-294,
@@ -267,6 +290,15 @@ typedef struct mvbt_op {
  * not defined); a call in which a batch writes one key twice, an op has an unknown
  * type, or the offsets disagree with num_ops is rejected on the device: nothing is
  * applied, results are left unchanged and MVBT_FLAG_BAD_BATCH is raised.
+ * The sorted form runs a large call in windows of about mvbt_set_batch_window operations
+ * (whole batches, at most 16 windows), each window's writes before its reads. When no
+ * snapshot taken through this API is live and no device view was handed out, the call
+ * reclaims between windows (the reference's DEBR frees inside its long mixed kernels,
+ * memory_reclaimer.hpp:244-401): every version copy retired so far is freed once the
+ * window's reads are done, so a call needs spare slots for about one window's copies, and
+ * each such reclamation appends an epoch to the memory timeline (mvbt_memory_series). Like
+ * mvbt_reclaim it must then not overlap readers of released snapshots on other streams;
+ * MVBT_NO_RECLAIM keeps every copy until mvbt_reclaim instead.
  * The tree clock advances by num_batches (also for a rejected call); *snapshot_base
  * (device pointer, may be NULL) receives base. The launch runs alone among this
  * handle's writers: it waits for writes issued earlier on other streams, and writes
@@ -277,6 +309,12 @@ mvbt_status mvbt_run_batches(mvbt_tree* tree, const mvbt_op* ops, uint64_t num_o
                              const uint64_t* batch_offsets, uint32_t num_batches,
                              uint32_t* results, mvbt_pair* pairs, uint64_t pair_slots, uint32_t L,
                              uint32_t* snapshot_base, void* stream, uint32_t flags);
+
+/* No counterpart in the reference (a capacity hint, like std::vector::reserve): sizes
+ * the batched form's scratch for calls of up to num_ops operations and num_batches
+ * batches and builds the launch graph of the current worker mode, so that later such
+ * calls allocate nothing and build nothing. Synchronous. */
+mvbt_status mvbt_reserve_batches(mvbt_tree* tree, uint64_t num_ops, uint32_t num_batches);
 
 /* No counterpart in the reference. For callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
@@ -313,6 +351,11 @@ mvbt_status mvbt_set_worker_waves(mvbt_tree* tree, uint32_t waves_per_cu);
  * by leaf group in batch order by the tile that owns the group, then every find and
  * range runs at its batch's snapshot in one pass. Same results in every mode. */
 mvbt_status mvbt_set_worker_mode(mvbt_tree* tree, uint32_t mode);
+
+/* Operations per window of the sorted form (autotuner knob; default 2^22; 0 = the whole
+ * call in one window). Smaller windows free copies sooner and shorten the version chains
+ * that reads walk; each window costs about 20 launches. No reference counterpart. */
+mvbt_status mvbt_set_batch_window(mvbt_tree* tree, uint64_t ops_per_window);
 
 /* ---- Device-side cooperative API ------------------------------------------ */
 

@@ -32,6 +32,7 @@ LATEST = 0xFFFFFFFF
 NOT_FOUND = 0xFFFFFFFF
 CONCURRENT = 0x1
 USE_GRAPH = 0x2
+NO_RECLAIM = 0x4
 OP_FIND, OP_INSERT, OP_ERASE, OP_RANGE = 0, 1, 2, 3
 FLAG_POOL_EXHAUSTED = 0x1
 FLAG_RETIRE_OVERFLOW = 0x2
@@ -55,6 +56,7 @@ EXPORTED_SYMBOLS = (
     "mvbt_export", "mvbt_node_width", "mvbt_kind_of", "mvbt_run_batches", "mvbt_set_worker_waves",
     "mvbt_memory_stats_async", "mvbt_device_view", "mvbt_bulk_build", "mvbt_concurrent_insert_range",
     "mvbt_concurrent_find_erase", "mvbt_export_dot", "mvbt_set_worker_mode", "mvbt_run_batches_noalloc",
+    "mvbt_reserve_batches", "mvbt_memory_series", "mvbt_set_batch_window",
 )
 
 
@@ -68,6 +70,11 @@ class _Config(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("node_width", ctypes.c_uint32),
                 ("pool_nodes", ctypes.c_uint64), ("retire_capacity", ctypes.c_uint64),
                 ("device", ctypes.c_int32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+# Fields of one memory-timeline record (mvbt_epoch_stat).
+EPOCH_STAT_FIELDS = ("epoch", "clock", "live_nodes", "allocated_total", "freed_nodes", "retired_pending",
+                     "reclaimed_total", "capacity_nodes")
 
 
 class _MemStats(ctypes.Structure):
@@ -138,6 +145,9 @@ def load_library():
         "mvbt_kind_of": ([vp, u32p], i32),
         "mvbt_run_batches": ([vp, vp, u64, vp, u32, vp, vp, u64, u32, vp, vp, u32], i32),
         "mvbt_run_batches_noalloc": ([vp, vp, u64, vp, u32, vp, vp, u64, u32, vp, vp], i32),
+        "mvbt_reserve_batches": ([vp, u64, u32], i32),
+        "mvbt_memory_series": ([vp, vp, u32, u32p], i32),
+        "mvbt_set_batch_window": ([vp, u64], i32),
         "mvbt_set_worker_waves": ([vp, u32], i32),
         "mvbt_set_worker_mode": ([vp, u32], i32),
         "mvbt_memory_stats_async": ([vp, vp, vp], i32),
@@ -336,6 +346,16 @@ class _TreeBase:
         _check(load_library().mvbt_memory_stats(self._h, ctypes.byref(s)), "memory_stats")
         return MemoryStats(**{f: getattr(s, f) for f, _ in _MemStats._fields_})
 
+    def memory_series(self, capacity: int = 4096):
+        """The memory timeline: one record per reclamation epoch (the latest
+        `capacity`, oldest first) with fields EPOCH_STAT_FIELDS."""
+        import numpy as np
+        buf = np.zeros((capacity, len(EPOCH_STAT_FIELDS)), dtype=np.uint64)
+        n = ctypes.c_uint32()
+        _check(load_library().mvbt_memory_series(self._h, ctypes.c_void_p(buf.ctypes.data), capacity,
+                                                 ctypes.byref(n)), "memory_series")
+        return np.rec.fromarrays(buf[: n.value].T, names=list(EPOCH_STAT_FIELDS))
+
     def memory_stats_async(self, out, stream=None):
         """Stream-ordered stats into `out` (8 uint64 words in device or pinned
         host memory: a torch int64 tensor of 8 elements)."""
@@ -398,7 +418,8 @@ class GpuVersionedBTree(_TreeBase):
 
     def run_batches(self, ops, batch_offsets, results=None, pairs=None, pair_slots: int = 0,
                     average_range_length: int = 0, snapshot_base=None, stream=None,
-                    use_graph: bool = False, num_ops: int | None = None, no_alloc: bool = False):
+                    use_graph: bool = False, num_ops: int | None = None, no_alloc: bool = False,
+                    no_reclaim: bool = False):
         """Batched form: `ops` is an int32 device tensor of shape (n, 4) holding
         {type, key, arg, slot} (OP_* types); `batch_offsets` an int64 device
         tensor of num_batches + 1 absolute op indices into `ops`; results go to
@@ -424,8 +445,17 @@ class GpuVersionedBTree(_TreeBase):
             self._h, _ptr(ops, 4, "ops"), n, _ptr(batch_offsets, 8, "batch_offsets"), nb,
             _ptr(results, 4, "results"), _ptr(pairs, 8, "pairs"), pair_slots,
             average_range_length, _ptr(snapshot_base, 4, "snapshot_base"), _stream(stream),
-            USE_GRAPH if use_graph else 0), "run_batches")
+            (USE_GRAPH if use_graph else 0) | (NO_RECLAIM if no_reclaim else 0)), "run_batches")
         return results
+
+    def reserve_batches(self, num_ops: int, num_batches: int):
+        """Sizes the batched form's scratch (and builds its launch graph) for
+        calls of up to num_ops operations and num_batches batches."""
+        _check(load_library().mvbt_reserve_batches(self._h, num_ops, num_batches), "reserve_batches")
+
+    def set_batch_window(self, ops_per_window: int):
+        """Operations per window of the sorted form (0: one window)."""
+        _check(load_library().mvbt_set_batch_window(self._h, ops_per_window), "set_batch_window")
 
     def set_worker_waves(self, waves_per_cu: int):
         _check(load_library().mvbt_set_worker_waves(self._h, waves_per_cu), "set_worker_waves")

@@ -75,6 +75,7 @@ def tune_batched(initial_keys, initial_values, groups, L: int, dev,
             tree.insert(dk, dv)
             tree.set_worker_mode(mode)
             tree.set_worker_waves(wv)
+            tree.reserve_batches(max(warm.nops, timed.nops), max(len(g) for g in groups))
             warm.run(tree, use_graph=True)
             tree.reclaim()
             torch.cuda.synchronize()

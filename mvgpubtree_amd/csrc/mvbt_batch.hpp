@@ -79,13 +79,27 @@ struct alignas(128) batch_job {
   uint32_t* sort_v[2];   // positions in w_rel (sort payloads), two buffers
   uint32_t* rd_find;     // find operations (index - first_op), any order
   uint32_t* rd_range;    // range operations (index - first_op), any order
-  uint32_t* seg;         // kSortSegments per-segment write counts
+  uint32_t* seg;         // per class (writes, finds, ranges) kSortSegments segment counts
   uint32_t* hist;        // kRadix * kSortSegments histogram + kRadix digit totals
   // counts of the current call (zeroed by batch_setup_kernel), one line
   uint32_t n_writes;
   uint32_t n_finds;
   uint32_t n_ranges;
-  uint32_t pad3[29];
+  uint32_t n_epochs;     // (leaf, batch) groups of the sorted writes: one planned copy slot each
+  uint32_t cp_valid;     // planned slots that exist (the rest: pool exhausted)
+  uint32_t cp_got;       // of them, taken from the free stack
+  uint32_t pad3[2];
+  long long cp_top;                 // free-stack top before the reservation
+  unsigned long long cp_bump;       // first never-used index of the reservation
+  unsigned long long cp_ring;       // first retire-ring position of the reservation
+  // windows of the sorted form: the call's batches in `windows` consecutive
+  // groups of about equal operation count; [w_lo, w_hi) = the current
+  // window's operations (index - first_op)
+  uint32_t windows;
+  uint32_t reclaim;
+  uint32_t w_lo;
+  uint32_t w_hi;
+  unsigned long long pad4[7];
 };
 constexpr uint32_t kFrontierCopies = 8;
 
@@ -100,6 +114,8 @@ struct batch_params {
   unsigned long long pair_slots;
   uint32_t* base_out;
   uint32_t hash_mask;
+  uint32_t windows;  // sorted form: windows of batches the call runs in (>= 1)
+  uint32_t reclaim;  // sorted form: free every retired copy after each window
 };
 
 // 1. Parameters in, clock range reserved, queue reset.
@@ -127,6 +143,10 @@ __global__ void batch_setup_kernel(batch_job* job, tree_state* st, batch_params 
   job->n_writes = 0;
   job->n_finds = 0;
   job->n_ranges = 0;
+  job->windows = p.windows > 0 ? p.windows : 1u;
+  job->reclaim = p.reclaim;
+  job->w_lo = 0;
+  job->w_hi = ok ? (uint32_t)(last - first) : 0u;
   if (!ok) atomic_or32(&st->error_flags, kErrBadBatch);
   for (uint32_t c = 0; c < kFrontierCopies; ++c) job->frontier[c * 32] = -1;
 }
@@ -614,73 +634,129 @@ __global__ void __launch_bounds__(256) batch_chunk_worker_kernel(tree_ptrs tp, b
 // No frontier and no per-batch wait: a call costs a few bulk passes however
 // many batches it holds. Results are identical to the per-batch path.
 
-// Wave-aggregated append of `v` to `list` (one atomic per wave).
-MVBT_DEV void append_wave(bool p, uint32_t* counter, uint32_t* list, uint32_t v) {
-  const uint64_t m = __builtin_amdgcn_ballot_w64(p);
-  if (!m) return;
-  const uint32_t lane = __lane_id(), leader = (uint32_t)__builtin_ctzll(m);
-  uint32_t base = 0;
-  if (lane == leader) base = atomic_add32(counter, (uint32_t)__builtin_popcountll(m));
-  base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
-  if (p) list[base + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1))] = v;
+// 5'. Window k of the sorted form: the batches whose first operation lies in
+// the k-th of `windows` equal slices of the call's operations (a batch is
+// never split). Windows run one after another; a read of a window sees the
+// writes of the earlier windows, which have been applied, and of its own
+// earlier batches, as in the whole-call form.
+MVBT_DEV uint32_t window_start(const batch_job* job, uint32_t k) {
+  const uint32_t nb = job->nb, K = job->windows;
+  if (k == 0) return 0;
+  if (k >= K) return nb;
+  const unsigned long long n = job->nops - job->first_op;
+  const unsigned long long target = job->first_op + (n * k) / K;
+  uint32_t lo = 0, hi = nb;  // the first batch starting at or after target
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (job->offsets[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
 }
 
-// 6a. Writes per segment of the operations (segments as in the sort).
-// A rejected call writes zero counts: the scan and the radix passes that
-// follow do not look at `bad`, and their sizes come from these counts.
+__global__ void batch_window_begin_kernel(batch_job* job, uint32_t k) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (ld32_coherent(&job->bad)) return;
+  const unsigned long long first = job->first_op;
+  job->w_lo = (uint32_t)(job->offsets[window_start(job, k)] - first);
+  job->w_hi = (uint32_t)(job->offsets[window_start(job, k + 1)] - first);
+  job->n_writes = 0;
+  job->n_finds = 0;
+  job->n_ranges = 0;
+  job->n_epochs = 0;
+}
+
+// End of a window: every read of the call at a snapshot below the next
+// window's first batch is done, so - when the call may reclaim (no snapshot
+// outside it is live) - every retired copy is garbage: the window's own copies
+// hold contents that only those reads could see. Frees them (appending an
+// epoch to the memory timeline) and merges parked slots either way; the
+// grid part is reclaim_copy_kernel.
+__global__ void batch_window_end_kernel(batch_job* job, tree_state* st, epoch_stat* series) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (ld32_coherent(&job->bad)) {
+    st->scratch[1] = 0;  // reclaim_copy_kernel then copies nothing
+    st->scratch[3] = 0;
+    return;
+  }
+  const bool r = job->reclaim != 0;
+  reclaim_prep(st, r ? ~0ull : 0ull, r ? series : nullptr);
+}
+
+// Operation classes of the split: writes (insert, erase), finds, ranges.
+MVBT_DEV uint32_t op_class(uint32_t ty) {
+  return (ty == kOpInsert || ty == kOpErase) ? 0u : ty == kOpFind ? 1u : ty == kOpRange ? 2u : 3u;
+}
+constexpr uint32_t kOpClasses = 3;
+
+// 6a. Operations of each class per segment (segments as in the sort):
+// seg[class * segments + segment]. A rejected call writes zero counts: the
+// scan and the radix passes that follow do not look at `bad`, and their
+// sizes come from these counts.
 __global__ void __launch_bounds__(kSortBlock) batch_split_count_kernel(batch_job* job) {
-  __shared__ uint32_t part[kSortBlock / 64];
-  const uint32_t n = ld32_coherent(&job->bad) ? 0u : (uint32_t)(job->nops - job->first_op);
-  const op_record* ops = job->ops + job->first_op;
+  __shared__ uint32_t part[kOpClasses][kSortBlock / 64];
+  const uint32_t n = ld32_coherent(&job->bad) ? 0u : job->w_hi - job->w_lo;
+  const op_record* ops = job->ops + job->first_op + job->w_lo;
   uint32_t a, b;
   segment_of(n, blockIdx.x, gridDim.x, a, b);
-  uint32_t c = 0;
+  uint32_t c[kOpClasses] = {0, 0, 0};
   for (uint32_t i = a + threadIdx.x; i < b; i += kSortBlock) {
-    const uint32_t ty = ops[i].type;
-    c += (ty == kOpInsert || ty == kOpErase) ? 1u : 0u;
+    const uint32_t k = op_class(ops[i].type);
+#pragma unroll
+    for (uint32_t q = 0; q < kOpClasses; ++q) c[q] += k == q ? 1u : 0u;
   }
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+#pragma unroll
+  for (uint32_t q = 0; q < kOpClasses; ++q) {
+    for (int o = 32; o > 0; o >>= 1) c[q] += __shfl_xor(c[q], o, 64);
+    if ((threadIdx.x & 63) == 0) part[q][threadIdx.x >> 6] = c[q];
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < kOpClasses) {
     uint32_t tot = 0;
-    for (uint32_t w = 0; w < kSortBlock / 64; ++w) tot += part[w];
-    job->seg[blockIdx.x] = tot;
+    for (uint32_t w = 0; w < kSortBlock / 64; ++w) tot += part[threadIdx.x][w];
+    job->seg[threadIdx.x * gridDim.x + blockIdx.x] = tot;
   }
 }
 
-// 6b (after radix_scan_kernel turned the counts into offsets and the total
-// into n_writes). The writes go to w_rel in operation order; finds and
-// ranges to their lists.
+// 6b (after radix_scan_kernel turned the counts into offsets and the class
+// totals into n_writes, n_finds, n_ranges). Each class goes to its list in
+// operation order (a stable scatter per segment; no shared counter): writes
+// to w_rel, finds to rd_find, ranges to rd_range.
 __global__ void __launch_bounds__(kSortBlock) batch_split_kernel(batch_job* job) {
   constexpr uint32_t kWaves = kSortBlock / 64;
-  __shared__ uint32_t wsum[kWaves];
+  __shared__ uint32_t wsum[kOpClasses][kWaves];
   if (ld32_coherent(&job->bad)) return;
-  const uint32_t n = (uint32_t)(job->nops - job->first_op);
-  const op_record* ops = job->ops + job->first_op;
-  uint32_t* w_rel = job->w_rel;
-  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t w_lo = job->w_lo, n = job->w_hi - w_lo;
+  const op_record* ops = job->ops + job->first_op + w_lo;
+  uint32_t* const lists[kOpClasses] = {job->w_rel, job->rd_find, job->rd_range};
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6, G = gridDim.x;
   const uint64_t below = (1ull << lane) - 1;
   uint32_t a, b;
-  segment_of(n, blockIdx.x, gridDim.x, a, b);
-  uint32_t run = job->seg[blockIdx.x];
+  segment_of(n, blockIdx.x, G, a, b);
+  uint32_t run[kOpClasses];
+#pragma unroll
+  for (uint32_t q = 0; q < kOpClasses; ++q) run[q] = job->seg[q * G + blockIdx.x];
   for (uint32_t c = a; c < b; c += kSortBlock) {
     const uint32_t i = c + tid;
-    const uint32_t ty = i < b ? ops[i].type : 0xFFu;
-    const bool isw = ty == kOpInsert || ty == kOpErase;
-    const uint64_t mw = __builtin_amdgcn_ballot_w64(isw);
-    if (lane == 0) wsum[w] = (uint32_t)__builtin_popcountll(mw);
-    append_wave(ty == kOpFind, &job->n_finds, job->rd_find, i);
-    append_wave(ty == kOpRange, &job->n_ranges, job->rd_range, i);
-    __syncthreads();
-    uint32_t pos = run + (uint32_t)__builtin_popcountll(mw & below), tot = 0;
+    const uint32_t k = i < b ? op_class(ops[i].type) : 3u;
+    uint64_t m[kOpClasses];
 #pragma unroll
-    for (uint32_t v = 0; v < kWaves; ++v) {
-      if (v < w) pos += wsum[v];
-      tot += wsum[v];
+    for (uint32_t q = 0; q < kOpClasses; ++q) {
+      m[q] = __builtin_amdgcn_ballot_w64(k == q);
+      if (lane == 0) wsum[q][w] = (uint32_t)__builtin_popcountll(m[q]);
     }
-    if (isw) w_rel[pos] = i;
-    run += tot;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < kOpClasses; ++q) {
+      uint32_t pos = run[q] + (uint32_t)__builtin_popcountll(m[q] & below), tot = 0;
+#pragma unroll
+      for (uint32_t v = 0; v < kWaves; ++v) {
+        if (v < w) pos += wsum[q][v];
+        tot += wsum[q][v];
+      }
+      if (k == q) lists[q][pos] = w_lo + i;
+      run[q] += tot;
+    }
     __syncthreads();
   }
 }
@@ -720,6 +796,115 @@ __global__ void __launch_bounds__(256) batch_route_kernel(tree_ptrs tp, batch_jo
   }
 }
 
+// 7'. Copy planning. Every (leaf, batch) group of the sorted writes may copy
+// its leaf once, so the groups are counted and numbered here (a stable
+// count / scan / assign over the sorted list) and one reservation takes a
+// slot for each from the free stack and the never-used frontier, and enters
+// them all in the retire ring: the write pass then copies into its group's
+// slot without touching the shared allocation and retirement counters. A
+// slot its group does not use (no change, or the per-key path took over)
+// is simply freed by the next reclamation.
+MVBT_DEV bool epoch_start(const batch_job* job, const uint32_t* hint, const uint32_t* pos, uint32_t i) {
+  if (i == 0) return true;
+  if (hint[i] != hint[i - 1]) return true;
+  const uint32_t* op_batch = job->op_batch;
+  const uint32_t* w_rel = job->w_rel;
+  return op_batch[w_rel[pos[i]]] != op_batch[w_rel[pos[i - 1]]];
+}
+
+__global__ void __launch_bounds__(kSortBlock) batch_epoch_count_kernel(batch_job* job, uint32_t which) {
+  __shared__ uint32_t part[kSortBlock / 64];
+  const uint32_t n = ld32_coherent(&job->bad) ? 0u : job->n_writes;
+  const uint32_t* hint = job->sort_k[which];
+  const uint32_t* pos = job->sort_v[which];
+  uint32_t a, b;
+  segment_of(n, blockIdx.x, gridDim.x, a, b);
+  uint32_t c = 0;
+  for (uint32_t i = a + threadIdx.x; i < b; i += kSortBlock) c += epoch_start(job, hint, pos, i) ? 1u : 0u;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < kSortBlock / 64; ++w) tot += part[w];
+    job->seg[blockIdx.x] = tot;
+  }
+}
+
+// (after radix_scan_kernel: seg holds each segment's first group number and
+// n_epochs the total) ep[i] = the group number of sorted write i.
+__global__ void __launch_bounds__(kSortBlock) batch_epoch_assign_kernel(batch_job* job, uint32_t which) {
+  constexpr uint32_t kWaves = kSortBlock / 64;
+  __shared__ uint32_t wsum[kWaves];
+  if (ld32_coherent(&job->bad)) return;
+  const uint32_t n = job->n_writes;
+  const uint32_t* hint = job->sort_k[which];
+  const uint32_t* pos = job->sort_v[which];
+  uint32_t* ep = job->sort_k[which ^ 1u];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint64_t below = (1ull << lane) - 1;
+  uint32_t a, b;
+  segment_of(n, blockIdx.x, gridDim.x, a, b);
+  uint32_t run = job->seg[blockIdx.x];  // groups started before this segment
+  for (uint32_t c = a; c < b; c += kSortBlock) {
+    const uint32_t i = c + tid;
+    const bool st = i < b && epoch_start(job, hint, pos, i);
+    const uint64_t m = __builtin_amdgcn_ballot_w64(st);
+    if (lane == 0) wsum[w] = (uint32_t)__builtin_popcountll(m);
+    __syncthreads();
+    uint32_t before = run + (uint32_t)__builtin_popcountll(m & below), tot = 0;
+#pragma unroll
+    for (uint32_t v = 0; v < kWaves; ++v) {
+      if (v < w) before += wsum[v];
+      tot += wsum[v];
+    }
+    // the group number is the count of starts at or before i, minus one
+    if (i < b) ep[i] = before + (st ? 1u : 0u) - 1u;
+    run += tot;
+    __syncthreads();
+  }
+}
+
+// One thread: reserves n_epochs slots (free stack first, then the
+// frontier) and as many retire-ring positions as slots exist.
+__global__ void batch_copy_reserve_kernel(batch_job* job, tree_ptrs tp) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (ld32_coherent(&job->bad)) return;
+  tree_state* st = tp.st;
+  const uint32_t E = job->n_epochs;
+  long long top = atomic_add_s64(&st->free_top, -(long long)E);
+  const uint32_t got = (uint32_t)(top < 0 ? 0 : (top > (long long)E ? (long long)E : top));
+  unsigned long long b = 0;
+  if (got < E) b = atomic_add64(&st->bump, (unsigned long long)(E - got));
+  const unsigned long long fresh = b < (unsigned long long)tp.cap_ ? (unsigned long long)tp.cap_ - b : 0ull;
+  const uint32_t valid = got + (uint32_t)((unsigned long long)(E - got) < fresh ? (E - got) : fresh);
+  if (valid < E) atomic_or32(&st->error_flags, kErrPoolExhausted);
+  const unsigned long long ring = atomic_add64(&st->retire_tail, (unsigned long long)valid);
+  job->cp_top = top;
+  job->cp_got = got;
+  job->cp_bump = b;
+  job->cp_valid = valid;
+  job->cp_ring = ring;
+}
+
+// Writes the slot of every group (kNone past the pool) to cslot and the
+// existing ones to the retire ring.
+__global__ void batch_copy_slots_kernel(batch_job* job, tree_ptrs tp, uint32_t which) {
+  if (ld32_coherent(&job->bad)) return;
+  const uint32_t E = job->n_epochs, got = job->cp_got, valid = job->cp_valid;
+  const long long top = job->cp_top;
+  const unsigned long long b = job->cp_bump, ring = job->cp_ring;
+  const unsigned long long mask = tp.retire_cap - 1;
+  uint32_t* cslot = job->sort_v[which ^ 1u];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < E; k += gridDim.x * blockDim.x) {
+    uint32_t slot = kNone;
+    if (k < got) slot = ld32_coherent(tp.free_stack + (top - 1 - (long long)k));
+    else if (k < valid) slot = (uint32_t)(b + (k - got));
+    cslot[k] = slot;
+    if (k < valid) tp.retire_ring[(ring + k) & mask] = slot;
+  }
+}
+
 // 8. The write pass over the sorted writes (buffers sort_k/v[which]).
 template <int NW>
 __global__ void __launch_bounds__(256) batch_sorted_write_kernel(tree_ptrs tp, batch_job* job, uint32_t which) {
@@ -737,6 +922,8 @@ __global__ void __launch_bounds__(256) batch_sorted_write_kernel(tree_ptrs tp, b
   const op_record* ops = job->ops + job->first_op;
   uint32_t* results = job->results + job->first_op;
   const uint32_t clock0 = job->clock0;
+  const uint32_t* ep = job->sort_k[which ^ 1u];     // group number of each sorted write
+  const uint32_t* cslot = job->sort_v[which ^ 1u];  // planned copy slot of each group
   write_ctx ctx = tv.begin_writes(t);
   ctx.exact = 1;
   ctx.track = 0;  // stamps are fixed per batch: clock0 + b + 1
@@ -770,6 +957,7 @@ __global__ void __launch_bounds__(256) batch_sorted_write_kernel(tree_ptrs tp, b
       const uint32_t b = act ? op_batch[rel] : 0u;
       const uint32_t cur = clock0 + b + 1;
       const uint32_t leaf = (act && h < tv.cap_) ? h : kNone;
+      const uint32_t planned = act ? cslot[ep[i]] : kNone;
       // lane r's group begins at glead (lane 0 or a group start of this sub-window)
       const uint64_t gs = t.ballot(start) | 1ull;
       const uint64_t upto = (r == 63) ? ~0ull : ((2ull << r) - 1);
@@ -787,8 +975,8 @@ __global__ void __launch_bounds__(256) batch_sorted_write_kernel(tree_ptrs tp, b
           uint64_t q = t.ballot(sel && op.type == (w == 0 ? kOpInsert : kOpErase));
           if (!q) continue;
           ctx.want = (uint32_t)__builtin_popcountll(q);
-          q = (w == 0) ? tv.template write_chunk_at<0>(op.key, op.arg, q, leaf, cur, res, ctx, t)
-                       : tv.template write_chunk_at<1>(op.key, 0u, q, leaf, cur, res, ctx, t);
+          q = (w == 0) ? tv.template write_chunk_at<0, true>(op.key, op.arg, q, leaf, cur, res, ctx, t, planned)
+                       : tv.template write_chunk_at<1, true>(op.key, 0u, q, leaf, cur, res, ctx, t, planned);
           while (q) {
             const uint32_t j = (uint32_t)lowest_bit(q);
             ctx.want = (uint32_t)__builtin_popcountll(q);

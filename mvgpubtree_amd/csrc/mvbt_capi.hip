@@ -53,6 +53,7 @@ struct mvbt_tree {
   uint32_t* free_stack = nullptr;
   uint32_t* spill = nullptr;
   uint32_t* retire_ring = nullptr;
+  epoch_stat* series = nullptr;         // memory timeline (versioned trees), kSeriesCap entries
   snapshot_record* snap_rec = nullptr;  // pinned host slot (synchronous snapshots)
   std::mutex mu;
   std::map<uint32_t, uint32_t> live;  // live snapshot id -> refcount
@@ -110,9 +111,12 @@ struct mvbt_tree {
   // 0: one operation per tile; 1: chunks (batch_chunk_worker_kernel);
   // 2: sorted (writes grouped by leaf, then every read at its snapshot)
   uint32_t worker_mode = 2;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  hipGraphNode_t graph_setup = nullptr;
+  // launch graphs: [0] for worker modes 0 and 1, [k] for the sorted form in k windows
+  static constexpr uint32_t kMaxWindows = 16;
+  hipGraph_t graph[kMaxWindows + 1] = {};
+  hipGraphExec_t graph_exec[kMaxWindows + 1] = {};
+  hipGraphNode_t graph_setup[kMaxWindows + 1] = {};
+  uint64_t window_ops = 1ull << 22;  // sorted form: operations per window (0: one window)
   bool external_writers = false;  // a device view was handed out
   bool fresh = true;              // no write, snapshot or device view yet (bulk build allowed)
 
@@ -293,9 +297,12 @@ bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 
 // Launches the two reclamation kernels: frees ring entries [head, end_pos)
 // (end_pos = 0 frees none) and returns parked spill slots to the free stack.
+// A reclamation (record) also appends an epoch to the memory timeline; a
+// spill merge alone does not.
 mvbt_status launch_maintenance(mvbt_tree* t, hipStream_t s, unsigned long long end_pos,
-                               const unsigned long long* end_from = nullptr) {
-  hipLaunchKernelGGL(reclaim_prep_kernel, dim3(1), dim3(64), 0, s, t->st, end_pos, end_from);
+                               const unsigned long long* end_from = nullptr, bool record = false) {
+  hipLaunchKernelGGL(reclaim_prep_kernel, dim3(1), dim3(64), 0, s, t->st, end_pos, end_from,
+                     record ? t->series : nullptr);
   MVBT_HIP(hipGetLastError());
   hipLaunchKernelGGL(reclaim_copy_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->st,
                      t->retire_ring, t->spill, t->free_stack);
@@ -521,8 +528,9 @@ constexpr uint64_t conflict_entries(uint64_t n) {
 
 // Entries of the sorted form's allocation before its histograms.
 constexpr uint64_t kSortedArrays = 8;
+constexpr uint64_t kSegWords = (uint64_t)kOpClasses * kSortSegments;  // the split's segment counts
 constexpr uint64_t sorted_scratch_words(uint64_t cap) {
-  return kSortedArrays * cap + kSortSegments + (uint64_t)kRadix * kSortSegments + kRadix;
+  return kSortedArrays * cap + kSegWords + (uint64_t)kRadix * kSortSegments + kRadix;
 }
 
 // ---- allocation-free batched form (mvbt_run_batches_noalloc) ----------------
@@ -595,7 +603,7 @@ mvbt_status ensure_batch_scratch(mvbt_tree* t, uint32_t nb, uint64_t nops) {
     uint32_t* sp[10];
     for (uint64_t k = 0; k < kSortedArrays; ++k) sp[k] = a + k * cap;
     sp[8] = a + kSortedArrays * cap;
-    sp[9] = sp[8] + kSortSegments;
+    sp[9] = sp[8] + kSegWords;
     static_assert(offsetof(batch_job, hist) == offsetof(batch_job, w_rel) + 9 * sizeof(void*), "sorted scratch layout");
     MVBT_HIP(hipMemcpy(&t->job->w_rel, sp, sizeof(sp), hipMemcpyHostToDevice));
     destroy_graph(t);  // the sort kernels of the graph take these pointers
@@ -656,12 +664,25 @@ uint32_t sort_passes(const mvbt_tree* t) {
   return (bits + 7) / 8;
 }
 
+// A sorted-form call may free version copies between its windows when no
+// snapshot outside it is live (none taken through this API, no device view
+// that could hold one) and the caller did not opt out (MVBT_NO_RECLAIM).
+bool may_reclaim_in_call(const mvbt_tree* t, uint32_t flags) {
+  return !(flags & MVBT_NO_RECLAIM) && t->live.empty() && !t->external_writers;
+}
+
+// Window k of the sorted form: begin, split, route, sort, copy planning,
+// write, find, range, end (reclamation when allowed, spill merge).
 template <class Add>
-mvbt_status sorted_sequence(mvbt_tree* t, Add&& add, batch_job* job, uint32_t* a, uint64_t cap) {
+mvbt_status sorted_window(mvbt_tree* t, Add& add, batch_job* job, uint32_t* a, uint64_t cap, uint32_t k) {
   uint32_t* sk[2] = {a + 2 * cap, a + 3 * cap};
+  {
+    void* args[] = {&job, &k};
+    if (hipError_t e = add(reinterpret_cast<void*>(&batch_window_begin_kernel), 1, 64, args)) return hip_fail(e);
+  }
   uint32_t* sv[2] = {a + 4 * cap, a + 5 * cap};
   uint32_t* seg = a + kSortedArrays * cap;
-  uint32_t* hist = seg + kSortSegments;
+  uint32_t* hist = seg + kSegWords;
   uint32_t* totals = hist + (uint64_t)kRadix * kSortSegments;
   uint32_t* n_writes = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(job) + offsetof(batch_job, n_writes));
   uint32_t segs = kSortSegments;
@@ -674,8 +695,11 @@ mvbt_status sorted_sequence(mvbt_tree* t, Add&& add, batch_job* job, uint32_t* a
       return hip_fail(e);
   }
   {
+    // one row per operation class: totals land in n_writes, n_finds, n_ranges
+    static_assert(offsetof(batch_job, n_ranges) == offsetof(batch_job, n_writes) + 2 * sizeof(uint32_t), "counts");
     void* args[] = {&seg, &segs, &n_writes};
-    if (hipError_t e = add(reinterpret_cast<void*>(&radix_scan_kernel), 1, kSortBlock, args)) return hip_fail(e);
+    if (hipError_t e = add(reinterpret_cast<void*>(&radix_scan_kernel), kOpClasses, kSortBlock, args))
+      return hip_fail(e);
   }
   {
     void* args[] = {&job};
@@ -705,6 +729,22 @@ mvbt_status sorted_sequence(mvbt_tree* t, Add&& add, batch_job* job, uint32_t* a
   }
   uint32_t which = passes & 1;  // the buffer the last pass wrote
   {
+    // copy planning: number the (leaf, batch) groups, reserve a slot for each
+    uint32_t* n_epochs = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(job) + offsetof(batch_job, n_epochs));
+    void* args[] = {&job, &which};
+    if (hipError_t e = add(reinterpret_cast<void*>(&batch_epoch_count_kernel), kSortSegments, kSortBlock, args))
+      return hip_fail(e);
+    void* sargs[] = {&seg, &segs, &n_epochs};
+    if (hipError_t e = add(reinterpret_cast<void*>(&radix_scan_kernel), 1, kSortBlock, sargs)) return hip_fail(e);
+    if (hipError_t e = add(reinterpret_cast<void*>(&batch_epoch_assign_kernel), kSortSegments, kSortBlock, args))
+      return hip_fail(e);
+    void* rargs[] = {&job, &tp};
+    if (hipError_t e = add(reinterpret_cast<void*>(&batch_copy_reserve_kernel), 1, 64, rargs)) return hip_fail(e);
+    void* cargs[] = {&job, &tp, &which};
+    if (hipError_t e = add(reinterpret_cast<void*>(&batch_copy_slots_kernel), grid, kBlock, cargs))
+      return hip_fail(e);
+  }
+  {
     // the write pass's occupancy is the tunable one (worker waves per CU)
     void* args[] = {&tp, &job, &which};
     if (hipError_t e = add(f.write, worker_blocks(t), kBlock, args)) return hip_fail(e);
@@ -717,6 +757,25 @@ mvbt_status sorted_sequence(mvbt_tree* t, Add&& add, batch_job* job, uint32_t* a
     void* args[] = {&tp, &job};
     if (hipError_t e = add(f.range, grid, kBlock, args)) return hip_fail(e);
   }
+  {
+    tree_state* st = t->st;
+    epoch_stat* series = t->series;
+    void* eargs[] = {&job, &st, &series};
+    if (hipError_t e = add(reinterpret_cast<void*>(&batch_window_end_kernel), 1, 64, eargs)) return hip_fail(e);
+    uint32_t* ring = t->retire_ring;
+    uint32_t* spill = t->spill;
+    uint32_t* fs = t->free_stack;
+    void* cargs[] = {&st, &ring, &spill, &fs};
+    if (hipError_t e = add(reinterpret_cast<void*>(&reclaim_copy_kernel), (uint32_t)t->num_cus * 4, kBlock, cargs))
+      return hip_fail(e);
+  }
+  return MVBT_OK;
+}
+
+template <class Add>
+mvbt_status sorted_sequence(mvbt_tree* t, Add&& add, batch_job* job, uint32_t* a, uint64_t cap, uint32_t windows) {
+  for (uint32_t k = 0; k < windows; ++k)
+    if (mvbt_status rc = sorted_window(t, add, job, a, cap, k)) return rc;
   return MVBT_OK;
 }
 
@@ -726,19 +785,30 @@ void* worker_kernel(const mvbt_tree* t) {
 }
 
 void destroy_graph(mvbt_tree* t) {
-  if (t->graph_exec) (void)hipGraphExecDestroy(t->graph_exec);
-  if (t->graph) (void)hipGraphDestroy(t->graph);
-  t->graph_exec = nullptr;
-  t->graph = nullptr;
-  t->graph_setup = nullptr;
+  for (uint32_t g = 0; g <= mvbt_tree::kMaxWindows; ++g) {
+    if (t->graph_exec[g]) (void)hipGraphExecDestroy(t->graph_exec[g]);
+    if (t->graph[g]) (void)hipGraphDestroy(t->graph[g]);
+    t->graph_exec[g] = nullptr;
+    t->graph[g] = nullptr;
+    t->graph_setup[g] = nullptr;
+  }
 }
 
-// Explicit six-node graph: setup -> clear -> count -> frontier -> worker -> finish.
-// Every node but setup reads its inputs from the device descriptor, so a call
-// only updates the setup node's parameters.
+// Windows of a sorted-form call: about window_ops operations each, at most
+// one per batch and kMaxWindows.
+uint32_t windows_for(const mvbt_tree* t, uint64_t num_ops, uint32_t num_batches) {
+  if (t->worker_mode != 2 || t->window_ops == 0) return 1;
+  const uint64_t k = (num_ops + t->window_ops - 1) / t->window_ops;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({k, num_batches, mvbt_tree::kMaxWindows}));
+}
+uint32_t graph_index(const mvbt_tree* t, uint32_t windows) { return t->worker_mode == 2 ? windows : 0; }
+
+// Explicit graph: setup -> clear -> count -> (frontier -> worker | the sorted
+// form's windows) -> finish. Every node but setup reads its inputs from the
+// device descriptor, so a call only updates the setup node's parameters.
 mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
-  destroy_graph(t);
-  MVBT_HIP(hipGraphCreate(&t->graph, 0));
+  const uint32_t gi = graph_index(t, bp.windows);
+  MVBT_HIP(hipGraphCreate(&t->graph[gi], 0));
   batch_job* job = t->job;
   tree_state* st = t->st;
   batch_params p = bp;
@@ -753,17 +823,17 @@ mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
     kp.sharedMemBytes = 0;
     kp.kernelParams = args;
     kp.extra = nullptr;
-    hipError_t e = hipGraphAddKernelNode(&node, t->graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
+    hipError_t e = hipGraphAddKernelNode(&node, t->graph[gi], prev ? &prev : nullptr, prev ? 1 : 0, &kp);
     prev = node;
     return e;
   };
   void* count_args[] = {&job, &st};
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_setup_kernel), 1, 64, setup_args));
-  t->graph_setup = node;
+  t->graph_setup[gi] = node;
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_clear_kernel), t->num_cus * 4, kBlock, job_args));
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_count_kernel), t->num_cus * 4, kBlock, count_args));
   if (t->worker_mode == 2) {
-    if (mvbt_status rc = sorted_sequence(t, add, t->job, t->b_sorted, t->b_sorted_cap)) return rc;
+    if (mvbt_status rc = sorted_sequence(t, add, t->job, t->b_sorted, t->b_sorted_cap, bp.windows)) return rc;
   } else {
     MVBT_HIP(add(reinterpret_cast<void*>(&batch_frontier_kernel), t->num_cus, kBlock, job_args));
     tree_ptrs tp = t->ptrs();
@@ -771,7 +841,7 @@ mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
     MVBT_HIP(add(worker_kernel(t), worker_blocks(t), kBlock, worker_args));
   }
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_finish_kernel), 1, 64, job_args));
-  MVBT_HIP(hipGraphInstantiate(&t->graph_exec, t->graph, nullptr, nullptr, 0));
+  MVBT_HIP(hipGraphInstantiate(&t->graph_exec[gi], t->graph[gi], nullptr, nullptr, 0));
   return MVBT_OK;
 }
 
@@ -824,6 +894,7 @@ mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
     if (t->retire_ring) (void)hipFree(t->retire_ring);
     if (t->snap_pos) (void)hipFree(t->snap_pos);
     if (t->snap_rec) (void)hipHostFree(t->snap_rec);
+    if (t->series) (void)hipFree(t->series);
   };
   const size_t node_bytes = (size_t)t->nw * sizeof(uint64_t);
   if (hipMalloc(&t->nodes, cfg.pool_nodes * node_bytes) != hipSuccess ||
@@ -832,7 +903,8 @@ mvbt_status mvbt_create(const mvbt_config* config, mvbt_tree** out) {
       hipMalloc(&t->spill, cfg.pool_nodes * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&t->retire_ring, cfg.retire_capacity * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&t->snap_pos, mvbt_tree::kSnapSlots * sizeof(unsigned long long)) != hipSuccess ||
-      hipHostMalloc(&t->snap_rec, sizeof(snapshot_record), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(&t->snap_rec, sizeof(snapshot_record), hipHostMallocDefault) != hipSuccess ||
+      (t->versioned && hipMalloc(&t->series, kSeriesCap * sizeof(epoch_stat)) != hipSuccess)) {
     (void)hipGetLastError();
     cleanup();
     return fail(MVBT_ERR_OUT_OF_MEMORY, "device allocation failed");
@@ -876,6 +948,7 @@ mvbt_status mvbt_destroy(mvbt_tree* t) {
   (void)hipFree(t->free_stack);
   (void)hipFree(t->spill);
   (void)hipFree(t->retire_ring);
+  if (t->series) (void)hipFree(t->series);
   (void)hipFree(t->snap_pos);
   (void)hipHostFree(t->snap_rec);
   for (auto& kv : t->write_ev) (void)hipEventDestroy(kv.second);
@@ -1030,7 +1103,7 @@ mvbt_status mvbt_reclaim(mvbt_tree* t, void* stream) {
     if (kv.first != s) MVBT_HIP(hipStreamWaitEvent(s, kv.second, 0));
   // a device-held edge is written by a snapshot kernel, possibly on another stream
   if (mvbt_status rc = after_clock(t, s)) return rc;
-  return launch_maintenance(t, s, end, end_from);
+  return launch_maintenance(t, s, end, end_from, true);
 }
 
 mvbt_status mvbt_memory_stats(mvbt_tree* t, mvbt_memstats* out) {
@@ -1049,6 +1122,27 @@ mvbt_status mvbt_memory_stats(mvbt_tree* t, mvbt_memstats* out) {
   out->retired_pending = s.retire_tail - s.retire_head;
   out->reclaimed_nodes = s.reclaimed;
   out->retired_total = s.retire_tail;
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_memory_series(mvbt_tree* t, mvbt_epoch_stat* out, uint32_t capacity, uint32_t* count) {
+  static_assert(sizeof(mvbt_epoch_stat) == sizeof(epoch_stat), "epoch stat layout");
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!count || (capacity && !out)) return fail(MVBT_ERR_INVALID_ARG, "null out or count");
+  *count = 0;
+  if (!t->versioned) return MVBT_OK;
+  device_guard g(t->device);
+  tree_state s;
+  if (mvbt_status rc = read_state(t, &s)) return rc;  // synchronises the device
+  const uint64_t have = std::min<uint64_t>(s.series_count, kSeriesCap);
+  const uint64_t n = std::min<uint64_t>(have, capacity);
+  std::vector<epoch_stat> all(kSeriesCap);
+  MVBT_HIP(hipMemcpy(all.data(), t->series, kSeriesCap * sizeof(epoch_stat), hipMemcpyDeviceToHost));
+  for (uint64_t i = 0; i < n; ++i) {  // the latest n, oldest first
+    const uint64_t e = s.series_count - n + i;
+    std::memcpy(out + i, &all[e % kSeriesCap], sizeof(epoch_stat));
+  }
+  *count = (uint32_t)n;
   return MVBT_OK;
 }
 
@@ -1189,8 +1283,11 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
   p.pair_slots = pairs ? pair_slots : 0;
   p.base_out = snapshot_base;
   p.hash_mask = (uint32_t)(conflict_entries(num_ops) - 1);
+  p.windows = windows_for(t, num_ops, num_batches);
+  p.reclaim = may_reclaim_in_call(t, flags) ? 1u : 0u;
   if (flags & MVBT_USE_GRAPH) {
-    if (!t->graph_exec) {
+    const uint32_t gi = graph_index(t, p.windows);
+    if (!t->graph_exec[gi]) {
       if (mvbt_status rc = build_graph(t, p)) return rc;
     }
     batch_job* job = t->job;
@@ -1201,8 +1298,8 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
     kp.gridDim = dim3(1);
     kp.blockDim = dim3(64);
     kp.kernelParams = args;
-    MVBT_HIP(hipGraphExecKernelNodeSetParams(t->graph_exec, t->graph_setup, &kp));
-    MVBT_HIP(hipGraphLaunch(t->graph_exec, s));
+    MVBT_HIP(hipGraphExecKernelNodeSetParams(t->graph_exec[gi], t->graph_setup[gi], &kp));
+    MVBT_HIP(hipGraphLaunch(t->graph_exec[gi], s));
   } else {
     hipLaunchKernelGGL(batch_setup_kernel, dim3(1), dim3(64), 0, s, t->job, t->st, p);
     hipLaunchKernelGGL(batch_clear_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job);
@@ -1211,7 +1308,7 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
       auto launch = [s](void* fn, unsigned grid, unsigned block, void** args) -> hipError_t {
         return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
       };
-      if (mvbt_status rc = sorted_sequence(t, launch, t->job, t->b_sorted, t->b_sorted_cap)) return rc;
+      if (mvbt_status rc = sorted_sequence(t, launch, t->job, t->b_sorted, t->b_sorted_cap, p.windows)) return rc;
     } else {
       hipLaunchKernelGGL(batch_frontier_kernel, dim3(t->num_cus), dim3(kBlock), 0, s, t->job);
       tree_ptrs tp = t->ptrs();
@@ -1226,6 +1323,27 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
   if (mvbt_status rc = clock_advanced(t, s, num_batches)) return rc;
   if (mvbt_status rc = isolated_done(t, s)) return rc;
   return after_write(t, s);
+}
+
+mvbt_status mvbt_reserve_batches(mvbt_tree* t, uint64_t num_ops, uint32_t num_batches) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  if (!t->versioned) return fail(MVBT_ERR_UNSUPPORTED, "the batched form needs a versioned tree");
+  if (num_batches > (1u << 30)) return fail(MVBT_ERR_INVALID_ARG, "too many batches");
+  if (num_ops >= (1ull << 31)) return fail(MVBT_ERR_INVALID_ARG, "num_ops must be below 2^31");
+  device_guard g(t->device);
+  std::lock_guard<std::mutex> lk(t->mu);
+  if (mvbt_status rc = ensure_batch_scratch(t, std::max<uint32_t>(num_batches, 1), std::max<uint64_t>(num_ops, 1)))
+    return rc;
+  const uint32_t kmax = windows_for(t, num_ops, std::max<uint32_t>(num_batches, 1));
+  for (uint32_t k = 1; k <= kmax; ++k) {
+    batch_params p{};
+    p.hash_mask = 1023;
+    p.windows = k;
+    if (!t->graph_exec[graph_index(t, k)])
+      if (mvbt_status rc = build_graph(t, p)) return rc;
+  }
+  MVBT_HIP(hipDeviceSynchronize());
+  return MVBT_OK;
 }
 
 mvbt_status mvbt_run_batches_noalloc(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
@@ -1262,7 +1380,7 @@ mvbt_status mvbt_run_batches_noalloc(mvbt_tree* t, const mvbt_op* ops, uint64_t 
   uint32_t* sorted = l.op_batch + kNoAllocMaxOps;
   for (uint64_t k = 0; k < kSortedArrays; ++k) l.sorted[k] = sorted + k * kNoAllocMaxOps;
   l.sorted[8] = sorted + kSortedArrays * kNoAllocMaxOps;
-  l.sorted[9] = l.sorted[8] + kSortSegments;
+  l.sorted[9] = l.sorted[8] + kSegWords;
   batch_params p{};
   p.ops = reinterpret_cast<const op_record*>(ops);
   p.nops = num_ops;
@@ -1274,6 +1392,10 @@ mvbt_status mvbt_run_batches_noalloc(mvbt_tree* t, const mvbt_op* ops, uint64_t 
   p.pair_slots = pairs ? pair_slots : 0;
   p.base_out = snapshot_base;
   p.hash_mask = (uint32_t)(conflict_entries(num_ops) - 1);
+  p.windows = 1;
+  if (t->window_ops) p.windows = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(
+      {(num_ops + t->window_ops - 1) / t->window_ops, num_batches, mvbt_tree::kMaxWindows}));
+  p.reclaim = may_reclaim_in_call(t, 0) ? 1u : 0u;
   hipLaunchKernelGGL(batch_bind_kernel, dim3(1), dim3(64), 0, s, job, l);
   hipLaunchKernelGGL(batch_setup_kernel, dim3(1), dim3(64), 0, s, job, t->st, p);
   hipLaunchKernelGGL(batch_clear_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, job);
@@ -1281,7 +1403,7 @@ mvbt_status mvbt_run_batches_noalloc(mvbt_tree* t, const mvbt_op* ops, uint64_t 
   auto launch = [s](void* fn, unsigned grid, unsigned block, void** args) -> hipError_t {
     return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
   };
-  if (mvbt_status rc = sorted_sequence(t, launch, job, sorted, kNoAllocMaxOps)) return rc;
+  if (mvbt_status rc = sorted_sequence(t, launch, job, sorted, kNoAllocMaxOps, p.windows)) return rc;
   hipLaunchKernelGGL(batch_finish_kernel, dim3(1), dim3(64), 0, s, job);
   MVBT_HIP(hipGetLastError());
   if (mvbt_status rc = clock_advanced(t, s, num_batches)) return rc;
@@ -1324,6 +1446,13 @@ mvbt_status mvbt_set_worker_mode(mvbt_tree* t, uint32_t mode) {
     destroy_graph(t);
     t->worker_mode = mode;
   }
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_set_batch_window(mvbt_tree* t, uint64_t ops_per_window) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  std::lock_guard<std::mutex> lk(t->mu);
+  t->window_ops = ops_per_window;
   return MVBT_OK;
 }
 

@@ -272,12 +272,8 @@ __global__ void snapshot_kernel(tree_state* st, snapshot_record* rec, unsigned l
 
 // Reclamation, phase 1 (one lane): frees retire-ring entries [head, end) by
 // moving them onto the free stack. Records the live count before freeing as a
-// peak candidate.
-__global__ void reclaim_prep_kernel(tree_state* st, unsigned long long end_pos,
-                                    const unsigned long long* end_from) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  // the edge recorded on the device by a stream-ordered snapshot
-  if (end_from) end_pos = __hip_atomic_load(end_from, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// peak candidate and, with `series`, appends the epoch to the memory timeline.
+MVBT_DEV void reclaim_prep(tree_state* st, unsigned long long end_pos, epoch_stat* series) {
   unsigned long long head = st->retire_head, tail = st->retire_tail;
   unsigned long long end = end_pos < tail ? end_pos : tail;
   if (end < head) end = head;
@@ -301,6 +297,27 @@ __global__ void reclaim_prep_kernel(tree_state* st, unsigned long long end_pos,
   st->scratch[3] = spilled;
   st->retire_head = end;
   st->reclaimed += count;
+  if (series) {
+    const unsigned long long e = st->series_count;
+    epoch_stat& x = series[e % kSeriesCap];
+    x.epoch = e;
+    x.clock = st->clock;
+    x.live_nodes = live - count;
+    x.allocated_total = live - count + st->reclaimed;
+    x.freed_nodes = count;
+    x.retired_pending = tail - end;
+    x.reclaimed_total = st->reclaimed;
+    x.capacity_nodes = st->capacity;
+    st->series_count = e + 1;
+  }
+}
+
+__global__ void reclaim_prep_kernel(tree_state* st, unsigned long long end_pos, const unsigned long long* end_from,
+                                    epoch_stat* series) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  // the edge recorded on the device by a stream-ordered snapshot
+  if (end_from) end_pos = __hip_atomic_load(end_from, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  reclaim_prep(st, end_pos, series);
 }
 
 // Stream-ordered memory statistics (same fields as mvbt_memstats).

@@ -63,7 +63,18 @@ struct alignas(128) tree_state {
   unsigned long long scratch[4];   // host<->device hand-off words
   uint32_t call_snapshot;          // snapshot reserved by the current mixed call
   uint32_t pad_call;
+  unsigned long long series_count;  // reclamation epochs recorded in the memory timeline
 };
+
+// One entry of the memory timeline (mvbt_epoch_stat in include/mvbt.h),
+// appended by every reclamation: the {usage, total allocated} pair the
+// reference records at each epoch advance (memory_reclaimer.hpp:387-398),
+// with the tree clock and the reclamation's own figures.
+struct epoch_stat {
+  unsigned long long epoch, clock, live_nodes, allocated_total, freed_nodes, retired_pending, reclaimed_total,
+      capacity_nodes;
+};
+constexpr uint32_t kSeriesCap = 4096;  // entries kept (a ring of the latest)
 
 // Result of a snapshot taken on the device: the id and the retire-ring
 // position at the epoch boundary (used by epoch reclamation).
@@ -1308,9 +1319,14 @@ struct tree_view : tree_ptrs {
   // The same with a stamp per lane (`cur`, equal for the lanes of one leaf):
   // the sorted batched form writes several batches' epochs in one chunk.
   // A tile that tracks the clock (c.track) uses c.cur for every lane.
-  template <int OP>
+  // PLANNED: `planned` (per lane, equal for the lanes of one leaf) is the
+  // version-copy slot reserved for the leaf's epoch ahead of the launch and
+  // already entered in the retire ring, so no copy allocates or retires here
+  // (kNone: the pool had no slot for it).
+  template <int OP, bool PLANNED = false>
   MVBT_DEV uint64_t write_chunk_at(uint32_t key, uint32_t value, uint64_t q, uint32_t leaf, uint32_t cur,
-                                   uint32_t& result, write_ctx& c, const tile_t& t) const {
+                                   uint32_t& result, write_ctx& c, const tile_t& t,
+                                   uint32_t planned = kNone) const {
     if constexpr (NW > 32) {
       return q;  // 64-pair nodes: register budget; per-key path
     } else {
@@ -1413,8 +1429,9 @@ struct tree_view : tree_ptrs {
 #pragma unroll
           for (int i = 0; i < kLeafBatch; ++i) {
             if (!((need_copy >> i) & 1)) continue;
-            const uint32_t slot = alloc(c, t, true);
+            const uint32_t slot = PLANNED ? t.shfl(planned, gl[i]) : alloc(c, t, true);
             if (slot == kNone) {  // pool exhausted: leave the leaf unchanged
+              if (PLANNED) flag(kErrPoolExhausted, t);
               ok &= ~(1u << i);
               changed &= ~(1u << i);
               need_copy &= ~(1u << i);
@@ -1423,7 +1440,7 @@ struct tree_view : tree_ptrs {
             node_t n;
             n.p = img[i];
             n.store_unlocked_copy(at(slot), t);
-            retire(c, slot, t);
+            if (!PLANNED) retire(c, slot, t);
             if (r == gl[i]) cp = slot;
           }
           if (need_copy) {

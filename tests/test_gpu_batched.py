@@ -236,3 +236,65 @@ def test_batched_read_only_and_write_only_calls(dev, mode):
     assert np.array_equal(res, want)
     assert tree.error_flags() == 4  # MVBT_FLAG_INVALID_KEY
     tree.close()
+
+
+@pytest.mark.parametrize("live_snapshot", [False, True])
+def test_sorted_form_in_windows_reclaims_between_them(dev, live_snapshot):
+    """A call cut into many windows (small window size): results identical to
+    the oracle. With no snapshot live the call frees the copies of each window
+    once its reads are done (one timeline epoch per window, nothing left
+    retired); a live snapshot keeps them all, and still reads its own state."""
+    w = small_config4(seed=140, num_batches=200)
+    tree = build_tree(w, dev, 32)
+    tree.set_worker_mode(2)
+    tree.set_batch_window(20_000)
+    s = tree.take_snapshot() if live_snapshot else None
+    epochs0 = len(tree.memory_series())
+    r = drivers.BatchedRunner(w.batches, w.range_capacity, dev, seed=5)
+    nwin = min(16, -(-r.nops // 20_000))
+    res = to_np(r.run(tree, use_graph=True))
+    want, wpairs, final = oracle.batch_expected(w.initial_keys, w.initial_values, r.ops_host, r.offsets_host,
+                                                w.range_capacity, r.slots)
+    bad = np.nonzero(res != want)[0]
+    assert bad.size == 0, (bad[:10], r.ops_host[bad[:10]], res[bad[:10]], want[bad[:10]])
+    assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
+    st = tree.memory_stats()
+    ser = tree.memory_series()
+    if live_snapshot:
+        assert len(ser) == epochs0
+        assert st.retired_pending > 0
+        q = w.initial_keys[:20000]
+        assert np.array_equal(to_np(tree.find(to_dev(q, dev), timestamp=s)),
+                              oracle.SortedState(w.initial_keys, w.initial_values).find(q))
+        tree.release_snapshot(s)
+    else:
+        assert len(ser) == epochs0 + nwin
+        assert st.retired_pending == 0
+        assert int(ser.freed_nodes[-nwin:].sum()) > 0
+    exp = tree.export()
+    assert np.array_equal(exp, final.keys.astype(np.uint64) | (final.values.astype(np.uint64) << np.uint64(32)))
+    assert tree.error_flags() == 0
+    tree.close()
+
+
+def test_sorted_form_fits_a_pool_that_holds_one_window_of_copies(dev):
+    """A stream whose version copies together exceed the pool's spare slots
+    runs to the end, bit-exact, when the call reclaims between windows."""
+    w = small_config4(seed=141, n_initial=100_000, num_batches=300, max_batch=8192)
+    n_ops = sum(b.num_ops for b in w.batches)
+    writes = sum(b.insert_keys.size + b.erase_keys.size for b in w.batches)
+    spare = writes // 3  # far fewer spare slots than the call makes copies
+    tree = mv.GpuVersionedBTree(pool_nodes=100_000 // 10 + 2_000 + spare, node_width=32)
+    tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
+    tree.set_worker_mode(2)
+    tree.set_batch_window(max(n_ops // 16, 1))
+    assert tree.memory_stats().free_nodes < writes
+    r = drivers.BatchedRunner(w.batches, w.range_capacity, dev, seed=6)
+    res = to_np(r.run(tree, use_graph=True))
+    assert tree.error_flags() == 0, hex(tree.error_flags())
+    want, wpairs, final = oracle.batch_expected(w.initial_keys, w.initial_values, r.ops_host, r.offsets_host,
+                                                w.range_capacity, r.slots)
+    assert np.array_equal(res, want)
+    assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
+    assert tree.validate_tree_structure()[0] == final.keys.size
+    tree.close()

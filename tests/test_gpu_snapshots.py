@@ -124,3 +124,41 @@ def test_ids_stay_in_sequence_after_device_view(dev):
         assert np.array_equal(to_np(tree.find(to_dev(keys, dev), timestamp=ts)), ref.find(keys, ts))
     assert tree.error_flags() == 0
     tree.close()
+
+
+def test_memory_series_records_every_reclamation(dev):
+    """Each reclamation appends one epoch to the device timeline: consecutive
+    epoch numbers, the clock at the time, live slots equal to memory_stats,
+    allocated = live + reclaimed, and the freed counts summing to the total."""
+    import torch
+    rng = np.random.default_rng(17)
+    keys = wl.unique_keys(rng, 60_000)
+    tree = mv.GpuVersionedBTree(pool_nodes=1 << 16)
+    tree.insert(to_dev(keys[:40_000], dev), to_dev(wl.value_of(keys[:40_000]), dev))
+    rows = []
+    for i in range(5):
+        s = tree.take_snapshot()
+        lo, hi = 40_000 + 4000 * i, 40_000 + 4000 * (i + 1)
+        tree.insert(to_dev(keys[lo:hi], dev), to_dev(wl.value_of(keys[lo:hi]), dev))
+        tree.erase(to_dev(keys[i * 3000:(i + 1) * 3000], dev))
+        tree.release_snapshot(s)
+        tree.reclaim()
+        torch.cuda.synchronize()
+        st = tree.memory_stats()
+        s2 = tree.take_snapshot()  # the clock after the reclamation
+        tree.release_snapshot(s2)
+        rows.append((st.live_nodes, st.reclaimed_nodes, s2))
+    ser = tree.memory_series()
+    assert len(ser) == 5
+    assert list(ser.epoch) == list(range(5))
+    for r, (live, reclaimed, clock_after) in zip(ser, rows):
+        assert r.live_nodes == live
+        assert r.reclaimed_total == reclaimed
+        assert r.allocated_total == r.live_nodes + r.reclaimed_total
+        assert r.retired_pending == 0
+        assert r.clock == clock_after  # no snapshot between the reclamation and s2
+    assert int(ser.freed_nodes.sum()) == rows[-1][1]
+    assert int(ser.freed_nodes.min()) > 0  # every batch made version copies
+    assert len(tree.memory_series(capacity=2)) == 2
+    assert tree.error_flags() == 0
+    tree.close()
