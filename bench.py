@@ -393,7 +393,9 @@ def batched_vs_plain(w, L, dev, nw: int, mode: int, waves: int, plain: bool = Tr
 
     n_ops = sum(b.num_ops for b in w.batches)
     n0 = w.initial_keys.size
-    pool = int(n0 / 4 * (16 / nw) + 2 * n_ops) + (1 << 20)
+    # spare slots: a fifth of the operation count for the sorted form, which
+    # reclaims between its windows; the persistent worker keeps every copy
+    pool = int(n0 / 4 * (16 / nw) + (n_ops / 5 if mode == 2 else 2 * n_ops)) + (1 << 20)
     dk, dv = dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev)
 
     def fresh_tree():

@@ -54,7 +54,7 @@ struct alignas(128) batch_job {
   unsigned long long pair_slots;  // capacity of `pairs` in slots of L pairs
   uint32_t* base_out;    // optional: receives clock0
   uint32_t clock0;
-  uint32_t hash_mask;    // conflict table size - 1 (a power of two >= 2 * ops)
+  uint32_t hash_mask;    // conflict table size - 1: a power of two >= 2 * writes (batch_mask_kernel)
   unsigned long long first_op;  // offsets[0]
   // library-owned scratch: capacity batches each, and the conflict table
   uint32_t* nwrites;
@@ -88,7 +88,8 @@ struct alignas(128) batch_job {
   uint32_t n_epochs;     // (leaf, batch) groups of the sorted writes: one planned copy slot each
   uint32_t cp_valid;     // planned slots that exist (the rest: pool exhausted)
   uint32_t cp_got;       // of them, taken from the free stack
-  uint32_t pad3[2];
+  uint32_t total_writes;  // write operations of the call (sizes the conflict table)
+  uint32_t hash_cap_mask;  // the conflict table's capacity - 1
   long long cp_top;                 // free-stack top before the reservation
   unsigned long long cp_bump;       // first never-used index of the reservation
   unsigned long long cp_ring;       // first retire-ring position of the reservation
@@ -130,7 +131,9 @@ __global__ void batch_setup_kernel(batch_job* job, tree_state* st, batch_params 
   job->pair_slots = p.pair_slots;
   job->base_out = p.base_out;
   job->clock0 = __hip_atomic_fetch_add(&st->clock, p.nb, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  job->hash_mask = p.hash_mask;
+  job->hash_cap_mask = p.hash_mask;
+  job->hash_mask = 1023u;
+  job->total_writes = 0;
   // Operation indices are absolute: batch b covers ops [offsets[b], offsets[b+1]),
   // results go to the same indices; offsets[nb] - offsets[0] must equal the op
   // count the caller passed (the size its buffers were checked against).
@@ -214,15 +217,15 @@ MVBT_DEV bool claim_write(const batch_job* job, unsigned long long i, uint32_t b
       if (cur == 0) return true;
     }
     const unsigned long long j = job->first_op + cur - 1u;
-    if (job->ops[j].key == key && batch_of(job->offsets, job->nb, j) == b) return false;
+    if (job->ops[j].key == key && job->op_batch[cur - 1u] == b) return false;
     h = (h + 1) & mask;
   }
   return true;
 }
 
-// 2. Writes per batch (one block per batch, grid-stride over batches), and
-// the call's checks: known op types, ordered offsets, distinct write keys
-// per batch.
+// 2. Writes per batch (one block per batch, grid-stride over batches), each
+// operation's batch, the call's write count, and the first checks: known op
+// types, ordered offsets.
 __global__ void batch_count_kernel(batch_job* job, tree_state* st) {
   __shared__ uint32_t partial[4];
   const uint32_t nb = job->nb;
@@ -236,7 +239,7 @@ __global__ void batch_count_kernel(batch_job* job, tree_state* st) {
       const bool w = o.type == kOpInsert || o.type == kOpErase;
       c += w ? 1u : 0u;
       job->op_batch[i - job->first_op] = b;
-      if (o.type > kOpRange || (w && valid_key(o.key) && !claim_write(job, i, b, o.key))) bad = true;
+      if (o.type > kOpRange) bad = true;
     }
     if (bad) {
       atomic_or32(&job->bad, 1u);
@@ -251,8 +254,40 @@ __global__ void batch_count_kernel(batch_job* job, tree_state* st) {
       for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) tot += partial[w];
       job->nwrites[b] = tot;
       job->done[b] = 0;
+      if (tot) atomic_add32(&job->total_writes, tot);
     }
     __syncthreads();
+  }
+}
+
+// 2'. The conflict table's size: the smallest power of two >= twice the
+// call's writes (at least 1024, at most its capacity), so a table that fits
+// the caches serves a call whose operations are mostly reads.
+__global__ void batch_mask_kernel(batch_job* job) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long e = 1024;
+  while (e < 2ull * job->total_writes) e *= 2;
+  const unsigned long long cap = (unsigned long long)job->hash_cap_mask + 1;
+  job->hash_mask = (uint32_t)((e < cap ? e : cap) - 1);
+}
+
+// 2''. (after batch_clear_kernel) Distinct write keys per batch: every write
+// of a valid key claims (batch, key) in the conflict table; a second claim
+// rejects the call.
+__global__ void batch_conflict_kernel(batch_job* job, tree_state* st) {
+  if (ld32_coherent(&job->bad)) return;
+  const unsigned long long first = job->first_op, n = job->nops - first;
+  bool bad = false;
+  for (unsigned long long r = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; r < n;
+       r += (unsigned long long)gridDim.x * blockDim.x) {
+    const op_record o = job->ops[first + r];
+    if ((o.type == kOpInsert || o.type == kOpErase) && valid_key(o.key) &&
+        !claim_write(job, first + r, job->op_batch[r], o.key))
+      bad = true;
+  }
+  if (bad) {
+    atomic_or32(&job->bad, 1u);
+    atomic_or32(&st->error_flags, kErrBadBatch);
   }
 }
 
@@ -1018,14 +1053,20 @@ __global__ void __launch_bounds__(kFindBlock) batch_sorted_find_kernel(tree_ptrs
     const uint32_t rel = act ? list[i] : 0u;
     const uint32_t key = act ? ops[rel].key : 0u;
     const uint32_t ts = act ? clock0 + op_batch[rel] : 0u;
-    uint32_t start = kNone, unused;
-    if (act && valid_key(key)) start = tv.route_lane(rc, key, unused);
     uint32_t result = kEmpty;
-    for (uint64_t q = t.ballot(act); q;) {
-      const uint32_t j = (uint32_t)lowest_bit(q);
-      q &= q - 1;
-      const uint32_t v = tv.template find<false>(t.shfl(key, j), t.shfl(ts, j), t, t.shfl(start, j));
-      if (t.rank() == j) result = v;
+    if constexpr (NW >= 32) {
+      // as in find_kernel: the pass-based engine keeps more walks in flight
+      result = tv.template read_chunk<false, route_cache<NW, kFindCacheBytes>, true>(key, 0u, ts, nullptr, 0u,
+                                                                                     t.ballot(act), t, &rc);
+    } else {
+      uint32_t start = kNone, unused;
+      if (act && valid_key(key)) start = tv.route_lane(rc, key, unused);
+      for (uint64_t q = t.ballot(act); q;) {
+        const uint32_t j = (uint32_t)lowest_bit(q);
+        q &= q - 1;
+        const uint32_t v = tv.template find<false>(t.shfl(key, j), t.shfl(ts, j), t, t.shfl(start, j));
+        if (t.rank() == j) result = v;
+      }
     }
     if (act) results[rel] = result;
   }

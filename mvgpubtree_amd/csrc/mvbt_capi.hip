@@ -803,7 +803,7 @@ uint32_t windows_for(const mvbt_tree* t, uint64_t num_ops, uint32_t num_batches)
 }
 uint32_t graph_index(const mvbt_tree* t, uint32_t windows) { return t->worker_mode == 2 ? windows : 0; }
 
-// Explicit graph: setup -> clear -> count -> (frontier -> worker | the sorted
+// Explicit graph: setup -> count -> mask -> clear -> conflict -> (frontier -> worker | the sorted
 // form's windows) -> finish. Every node but setup reads its inputs from the
 // device descriptor, so a call only updates the setup node's parameters.
 mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
@@ -830,8 +830,10 @@ mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
   void* count_args[] = {&job, &st};
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_setup_kernel), 1, 64, setup_args));
   t->graph_setup[gi] = node;
-  MVBT_HIP(add(reinterpret_cast<void*>(&batch_clear_kernel), t->num_cus * 4, kBlock, job_args));
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_count_kernel), t->num_cus * 4, kBlock, count_args));
+  MVBT_HIP(add(reinterpret_cast<void*>(&batch_mask_kernel), 1, 64, job_args));
+  MVBT_HIP(add(reinterpret_cast<void*>(&batch_clear_kernel), t->num_cus * 4, kBlock, job_args));
+  MVBT_HIP(add(reinterpret_cast<void*>(&batch_conflict_kernel), t->num_cus * 8, kBlock, count_args));
   if (t->worker_mode == 2) {
     if (mvbt_status rc = sorted_sequence(t, add, t->job, t->b_sorted, t->b_sorted_cap, bp.windows)) return rc;
   } else {
@@ -1302,8 +1304,10 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
     MVBT_HIP(hipGraphLaunch(t->graph_exec[gi], s));
   } else {
     hipLaunchKernelGGL(batch_setup_kernel, dim3(1), dim3(64), 0, s, t->job, t->st, p);
-    hipLaunchKernelGGL(batch_clear_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job);
     hipLaunchKernelGGL(batch_count_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job, t->st);
+    hipLaunchKernelGGL(batch_mask_kernel, dim3(1), dim3(64), 0, s, t->job);
+    hipLaunchKernelGGL(batch_clear_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job);
+    hipLaunchKernelGGL(batch_conflict_kernel, dim3(t->num_cus * 8), dim3(kBlock), 0, s, t->job, t->st);
     if (t->worker_mode == 2) {
       auto launch = [s](void* fn, unsigned grid, unsigned block, void** args) -> hipError_t {
         return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
@@ -1398,8 +1402,10 @@ mvbt_status mvbt_run_batches_noalloc(mvbt_tree* t, const mvbt_op* ops, uint64_t 
   p.reclaim = may_reclaim_in_call(t, 0) ? 1u : 0u;
   hipLaunchKernelGGL(batch_bind_kernel, dim3(1), dim3(64), 0, s, job, l);
   hipLaunchKernelGGL(batch_setup_kernel, dim3(1), dim3(64), 0, s, job, t->st, p);
-  hipLaunchKernelGGL(batch_clear_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, job);
   hipLaunchKernelGGL(batch_count_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, job, t->st);
+  hipLaunchKernelGGL(batch_mask_kernel, dim3(1), dim3(64), 0, s, job);
+  hipLaunchKernelGGL(batch_clear_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, job);
+  hipLaunchKernelGGL(batch_conflict_kernel, dim3(t->num_cus * 8), dim3(kBlock), 0, s, job, t->st);
   auto launch = [s](void* fn, unsigned grid, unsigned block, void** args) -> hipError_t {
     return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
   };

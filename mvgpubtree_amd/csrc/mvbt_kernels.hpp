@@ -65,16 +65,23 @@ __global__ void __launch_bounds__(kFindBlock) find_kernel(tree_view<NW, VER> tv,
     uint32_t key = active ? keys[i] : 0u;
     uint32_t result = kEmpty;
     uint64_t q = t.ballot(active);
-    uint32_t start = kNone, unused;  // each lane routes its key through the cached levels
-    if (active && valid_key(key)) start = tv.route_lane(rc, key, unused);
-    // Key by key: a find is one walk to a leaf whose upper levels stay in
-    // the L1, and the small register footprint keeps 8 waves per SIMD in
-    // flight; the pass-based engine (read_chunk) measured slower here.
-    while (q) {
-      uint32_t j = (uint32_t)lowest_bit(q);
-      q &= q - 1;
-      uint32_t r = tv.template find<CONC>(t.shfl(key, j), ts, t, t.shfl(start, j));
-      if (t.rank() == j) result = r;
+    if constexpr (!CONC && NW >= 32) {
+      // 32- and 64-pair tiles: a wave holds only two (one) of them, so one
+      // walk per tile leaves too few loads in flight; the pass-based engine
+      // advances kReadGroup walks of the tile per pass instead.
+      result = tv.template read_chunk<false>(key, 0u, ts, nullptr, 0u, q, t, &rc);
+    } else {
+      uint32_t start = kNone, unused;  // each lane routes its key through the cached levels
+      if (active && valid_key(key)) start = tv.route_lane(rc, key, unused);
+      // Key by key: a find is one walk to a leaf whose upper levels stay in
+      // the L1, and the small register footprint keeps 8 waves per SIMD in
+      // flight; at 16-pair tiles the pass-based engine measured slower.
+      while (q) {
+        uint32_t j = (uint32_t)lowest_bit(q);
+        q &= q - 1;
+        uint32_t r = tv.template find<CONC>(t.shfl(key, j), ts, t, t.shfl(start, j));
+        if (t.rank() == j) result = r;
+      }
     }
     if (active) values[i] = result;
   }

@@ -127,14 +127,22 @@ def test_config4_full_size(dev):
     n_ops = sum(b.num_ops for b in w.batches)
     note(f"config4: {n_ops} ops generated in {time.time() - t0:.1f}s")
     nw = 32
-    tree = mv.GpuVersionedBTree(pool_nodes=int(40_000_000 / 4 * (16 / nw) + 2 * n_ops) + (1 << 20), node_width=nw)
+    # spare slots: a fifth of the operation count (the call reclaims between
+    # its windows, so it never holds more than about one window of copies)
+    tree = mv.GpuVersionedBTree(pool_nodes=int(40_000_000 / 4 * (16 / nw) + n_ops / 5), node_width=nw)
     tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
     tree.set_worker_mode(2)
+    spare = tree.memory_stats().free_nodes
+    assert spare < n_ops / 2
+    e0 = len(tree.memory_series())
     r = drivers.BatchedRunner(w.batches, L, dev)
     r.run(tree, use_graph=True)
     sync()
-    note(f"config4: ran ({time.time() - t0:.1f}s)")
+    note(f"config4: ran ({time.time() - t0:.1f}s), {spare / n_ops:.3f} spare slots per op")
     assert tree.error_flags() == 0, hex(tree.error_flags())
+    ser = tree.memory_series()[e0:]
+    assert len(ser) >= 8  # one reclamation per window
+    assert tree.memory_stats().retired_pending == 0
     ks = np.sort(w.initial_keys)
     replay = oracle.WriteOnceReplay(ks, wl.value_of(ks), r.ops_host, r.offsets_host)
     compare_batched(r.results, r.pairs, L, r.ops_host, r.offsets_host, replay)
