@@ -14,8 +14,9 @@ Prints one JSON line with the contract fields plus:
   cpu_baseline  the oracle (std::map restatement of the reference's serial
                 validation path) on one batch of the same workload on this
                 host: one thread, and key-range shards over the host's cores;
-  breakdown     per-operation rates, plus 1M finds (50% hits) on a tree of the
-                same 40M keys with 16-pair nodes (--find-node-width).
+  breakdown     per-operation rates, plus 1M finds (50% hits) on the measured
+                tree (--find-node-width picks another width: a fresh tree of the
+                same 40M keys).
 """
 from __future__ import annotations
 
@@ -637,8 +638,9 @@ def main():
     ap.add_argument("--node-width", type=int, default=32,
                     help="config 2: node width of the measured tree (32 pairs measured fastest for "
                          "insert + range-32 batches)")
-    ap.add_argument("--find-node-width", type=int, default=16,
-                    help="config 2: node width of the tree the find breakdown runs on")
+    ap.add_argument("--find-node-width", type=int, default=32,
+                    help="config 2: node width of the tree the find breakdown runs on (default: the "
+                         "measured tree itself)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
                     help="2: headline mixed insert/range batches; 3: erase/find batches with epoch "
@@ -658,7 +660,7 @@ def main():
     ap.add_argument("--waves", default="8,16,24,32", help="config 4: waves/CU to tune")
     ap.add_argument("--no-extras", action="store_true",
                     help="default run: skip the config-4 and config-5 legs of the breakdown")
-    ap.add_argument("--extras-qps", default="8000,32000", help="default run: config-5 QPS levels")
+    ap.add_argument("--extras-qps", default="8000,32000,128000,512000", help="default run: config-5 QPS levels")
     ap.add_argument("--extras-requests", type=int, default=1000, help="default run: config-5 requests per level")
     ap.add_argument("--c4-node-width", type=int, default=32, help="default run: config-4 node width")
     ap.add_argument("--c4-waves", type=int, default=16, help="default run: config-4 worker waves per CU")

@@ -319,7 +319,7 @@ mvbt_status mvbt_reserve_batches(mvbt_tree* tree, uint64_t num_ops, uint32_t num
 /* No counterpart in the reference. For callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
  * mvbt_run_batches in the sorted form (worker mode 2) with every scratch array in one
- * static device array of the library (about 24 MB on each device it runs on, held
+ * static device array of the library (about 37 MB on each device it runs on, held
  * until the process ends) instead of handle-owned allocations. Same arguments,
  * results and rejections as mvbt_run_batches (no flags: launched directly, no graph).
  * Limits: num_ops <= 524288 (2^19) and num_batches <= 16384 (2^14) per call, else

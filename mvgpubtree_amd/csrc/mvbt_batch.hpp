@@ -81,6 +81,9 @@ struct alignas(128) batch_job {
   uint32_t* rd_range;    // range operations (index - first_op), any order
   uint32_t* seg;         // per class (writes, finds, ranges) kSortSegments segment counts
   uint32_t* hist;        // kRadix * kSortSegments histogram + kRadix digit totals
+  // the sorted writes' fields in sorted order (batch_gather_kernel): key,
+  // value, index - first_op (bit 31: erase), batch, parent hint, copy slot
+  uint32_t* gat[6];
   // counts of the current call (zeroed by batch_setup_kernel), one line
   uint32_t n_writes;
   uint32_t n_finds;
@@ -163,7 +166,7 @@ struct scratch_layout {
   int32_t* advance;
   uint32_t* hash;
   uint32_t* op_batch;
-  uint32_t* sorted[10];  // w_rel, w_par, sort_k[0..1], sort_v[0..1], rd_find, rd_range, seg, hist
+  uint32_t* sorted[16];  // w_rel, w_par, sort_k[0..1], sort_v[0..1], rd_find, rd_range, seg, hist, gat[0..5]
 };
 __global__ void batch_bind_kernel(batch_job* job, scratch_layout l) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -182,6 +185,7 @@ __global__ void batch_bind_kernel(batch_job* job, scratch_layout l) {
   job->rd_range = l.sorted[7];
   job->seg = l.sorted[8];
   job->hist = l.sorted[9];
+  for (int k = 0; k < 6; ++k) job->gat[k] = l.sorted[10 + k];
 }
 
 // 1'. Clears the conflict table (a fill kernel, so the launch graph holds no
@@ -940,6 +944,30 @@ __global__ void batch_copy_slots_kernel(batch_job* job, tree_ptrs tp, uint32_t w
   }
 }
 
+// 7''. Gathers each sorted write's fields into sorted order, so the write
+// pass reads them with coalesced loads instead of chasing the position, the
+// operation and its batch one dependent load after another.
+__global__ void __launch_bounds__(256) batch_gather_kernel(batch_job* job, uint32_t which) {
+  if (ld32_coherent(&job->bad)) return;
+  const uint32_t n = job->n_writes;
+  const uint32_t* pos = job->sort_v[which];
+  const uint32_t* ep = job->sort_k[which ^ 1u];
+  const uint32_t* cslot = job->sort_v[which ^ 1u];
+  const op_record* ops = job->ops + job->first_op;
+  uint32_t* const* g = job->gat;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t pv = pos[i];
+    const uint32_t rel = job->w_rel[pv];
+    const op_record o = ops[rel];
+    g[0][i] = o.key;
+    g[1][i] = o.arg;
+    g[2][i] = rel | (o.type == kOpErase ? 0x80000000u : 0u);
+    g[3][i] = job->op_batch[rel];
+    g[4][i] = job->w_par[pv];
+    g[5][i] = cslot[ep[i]];
+  }
+}
+
 // 8. The write pass over the sorted writes (buffers sort_k/v[which]).
 template <int NW>
 __global__ void __launch_bounds__(256) batch_sorted_write_kernel(tree_ptrs tp, batch_job* job, uint32_t which) {
@@ -950,15 +978,9 @@ __global__ void __launch_bounds__(256) batch_sorted_write_kernel(tree_ptrs tp, b
   const uint32_t r = t.rank();
   const uint32_t n = job->n_writes;
   const uint32_t* hint = job->sort_k[which];
-  const uint32_t* pos = job->sort_v[which];
-  const uint32_t* w_rel = job->w_rel;
-  const uint32_t* w_par = job->w_par;
-  const uint32_t* op_batch = job->op_batch;
-  const op_record* ops = job->ops + job->first_op;
+  const uint32_t* const* g = job->gat;  // the writes' fields in sorted order
   uint32_t* results = job->results + job->first_op;
   const uint32_t clock0 = job->clock0;
-  const uint32_t* ep = job->sort_k[which ^ 1u];     // group number of each sorted write
-  const uint32_t* cslot = job->sort_v[which ^ 1u];  // planned copy slot of each group
   write_ctx ctx = tv.begin_writes(t);
   ctx.exact = 1;
   ctx.track = 0;  // stamps are fixed per batch: clock0 + b + 1
@@ -984,15 +1006,20 @@ __global__ void __launch_bounds__(256) batch_sorted_write_kernel(tree_ptrs tp, b
       }
       if (!mine) break;
       const bool act = (mine >> r) & 1;
-      const uint32_t pv = act ? pos[i] : 0u;
-      const uint32_t rel = act ? w_rel[pv] : 0u;
-      const uint32_t par = act ? w_par[pv] : kNone;
       op_record op{};
-      if (act) op = ops[rel];
-      const uint32_t b = act ? op_batch[rel] : 0u;
+      uint32_t rel = 0, b = 0, par = kNone, planned = kNone;
+      if (act) {
+        op.key = g[0][i];
+        op.arg = g[1][i];
+        const uint32_t rt = g[2][i];
+        rel = rt & 0x7FFFFFFFu;
+        op.type = (rt >> 31) ? kOpErase : kOpInsert;
+        b = g[3][i];
+        par = g[4][i];
+        planned = g[5][i];
+      }
       const uint32_t cur = clock0 + b + 1;
       const uint32_t leaf = (act && h < tv.cap_) ? h : kNone;
-      const uint32_t planned = act ? cslot[ep[i]] : kNone;
       // lane r's group begins at glead (lane 0 or a group start of this sub-window)
       const uint64_t gs = t.ballot(start) | 1ull;
       const uint64_t upto = (r == 63) ? ~0ull : ((2ull << r) - 1);
@@ -1034,7 +1061,8 @@ __global__ void __launch_bounds__(256) batch_sorted_write_kernel(tree_ptrs tp, b
 
 // 9. Finds, each at its batch's snapshot (no writer runs).
 template <int NW>
-__global__ void __launch_bounds__(kFindBlock) batch_sorted_find_kernel(tree_ptrs tp, const batch_job* job) {
+__global__ void __launch_bounds__(kFindBlock) __attribute__((amdgpu_waves_per_eu(8)))
+batch_sorted_find_kernel(tree_ptrs tp, const batch_job* job) {
   if (ld32_coherent(&job->bad)) return;
   const tree_view<NW, true> tv(tp);
   tile<NW> t;
@@ -1056,8 +1084,8 @@ __global__ void __launch_bounds__(kFindBlock) batch_sorted_find_kernel(tree_ptrs
     uint32_t result = kEmpty;
     if constexpr (NW >= 32) {
       // as in find_kernel: the pass-based engine keeps more walks in flight
-      result = tv.template read_chunk<false, route_cache<NW, kFindCacheBytes>, true>(key, 0u, ts, nullptr, 0u,
-                                                                                     t.ballot(act), t, &rc);
+      result = tv.template read_chunk<false, route_cache<NW, kFindCacheBytes>, true, tree_view<NW, true>::kFindGroup>(
+          key, 0u, ts, nullptr, 0u, t.ballot(act), t, &rc);
     } else {
       uint32_t start = kNone, unused;
       if (act && valid_key(key)) start = tv.route_lane(rc, key, unused);

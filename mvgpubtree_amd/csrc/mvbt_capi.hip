@@ -527,7 +527,7 @@ constexpr uint64_t conflict_entries(uint64_t n) {
 }
 
 // Entries of the sorted form's allocation before its histograms.
-constexpr uint64_t kSortedArrays = 8;
+constexpr uint64_t kSortedArrays = 14;  // 8 working arrays + 6 gathered write fields
 constexpr uint64_t kSegWords = (uint64_t)kOpClasses * kSortSegments;  // the split's segment counts
 constexpr uint64_t sorted_scratch_words(uint64_t cap) {
   return kSortedArrays * cap + kSegWords + (uint64_t)kRadix * kSortSegments + kRadix;
@@ -536,7 +536,7 @@ constexpr uint64_t sorted_scratch_words(uint64_t cap) {
 // ---- allocation-free batched form (mvbt_run_batches_noalloc) ----------------
 // Limits of one call, and the static scratch they need: the job descriptor,
 // three per-batch counters, the conflict table, each op's batch, and the
-// sorted form's arrays (about 24 MB per device).
+// sorted form's arrays (about 37 MB per device).
 constexpr uint64_t kNoAllocMaxOps = 1ull << 19;
 constexpr uint32_t kNoAllocMaxBatches = 1u << 14;
 constexpr uint64_t kNoAllocHash = conflict_entries(kNoAllocMaxOps);
@@ -599,12 +599,13 @@ mvbt_status ensure_batch_scratch(mvbt_tree* t, uint32_t nb, uint64_t nops) {
     MVBT_HIP(hipMalloc(&t->b_sorted, sorted_scratch_words(cap) * sizeof(uint32_t)));
     t->b_sorted_cap = cap;
     uint32_t* a = t->b_sorted;
-    // w_rel, w_par, sort_k[0..1], sort_v[0..1], rd_find, rd_range, seg, hist
-    uint32_t* sp[10];
-    for (uint64_t k = 0; k < kSortedArrays; ++k) sp[k] = a + k * cap;
+    // w_rel, w_par, sort_k[0..1], sort_v[0..1], rd_find, rd_range, seg, hist, gat[0..5]
+    uint32_t* sp[16];
+    for (uint64_t k = 0; k < 8; ++k) sp[k] = a + k * cap;
     sp[8] = a + kSortedArrays * cap;
     sp[9] = sp[8] + kSegWords;
-    static_assert(offsetof(batch_job, hist) == offsetof(batch_job, w_rel) + 9 * sizeof(void*), "sorted scratch layout");
+    for (uint64_t k = 0; k < 6; ++k) sp[10 + k] = a + (8 + k) * cap;
+    static_assert(offsetof(batch_job, gat) == offsetof(batch_job, w_rel) + 10 * sizeof(void*), "sorted scratch layout");
     MVBT_HIP(hipMemcpy(&t->job->w_rel, sp, sizeof(sp), hipMemcpyHostToDevice));
     destroy_graph(t);  // the sort kernels of the graph take these pointers
   }
@@ -743,6 +744,7 @@ mvbt_status sorted_window(mvbt_tree* t, Add& add, batch_job* job, uint32_t* a, u
     void* cargs[] = {&job, &tp, &which};
     if (hipError_t e = add(reinterpret_cast<void*>(&batch_copy_slots_kernel), grid, kBlock, cargs))
       return hip_fail(e);
+    if (hipError_t e = add(reinterpret_cast<void*>(&batch_gather_kernel), grid, kBlock, args)) return hip_fail(e);
   }
   {
     // the write pass's occupancy is the tunable one (worker waves per CU)
@@ -1382,9 +1384,10 @@ mvbt_status mvbt_run_batches_noalloc(mvbt_tree* t, const mvbt_op* ops, uint64_t 
   l.hash = w + 3 * kNoAllocMaxBatches;
   l.op_batch = l.hash + kNoAllocHash;
   uint32_t* sorted = l.op_batch + kNoAllocMaxOps;
-  for (uint64_t k = 0; k < kSortedArrays; ++k) l.sorted[k] = sorted + k * kNoAllocMaxOps;
+  for (uint64_t k = 0; k < 8; ++k) l.sorted[k] = sorted + k * kNoAllocMaxOps;
   l.sorted[8] = sorted + kSortedArrays * kNoAllocMaxOps;
   l.sorted[9] = l.sorted[8] + kSegWords;
+  for (uint64_t k = 0; k < 6; ++k) l.sorted[10 + k] = sorted + (8 + k) * kNoAllocMaxOps;
   batch_params p{};
   p.ops = reinterpret_cast<const op_record*>(ops);
   p.nops = num_ops;

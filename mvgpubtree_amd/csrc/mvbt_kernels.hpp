@@ -53,7 +53,8 @@ __global__ void init_kernel(tree_view<NW, VER> tv) {
 }
 
 template <int NW, bool VER, bool CONC>
-__global__ void __launch_bounds__(kFindBlock) find_kernel(tree_view<NW, VER> tv, const uint32_t* keys,
+__global__ void __launch_bounds__(kFindBlock) __attribute__((amdgpu_waves_per_eu(8)))
+find_kernel(tree_view<NW, VER> tv, const uint32_t* keys,
                                                           uint32_t* values, uint64_t n, uint32_t ts) {
   tile<NW> t;
   chunking<NW> c;
@@ -69,7 +70,9 @@ __global__ void __launch_bounds__(kFindBlock) find_kernel(tree_view<NW, VER> tv,
       // 32- and 64-pair tiles: a wave holds only two (one) of them, so one
       // walk per tile leaves too few loads in flight; the pass-based engine
       // advances kReadGroup walks of the tile per pass instead.
-      result = tv.template read_chunk<false>(key, 0u, ts, nullptr, 0u, q, t, &rc);
+      using rc_t = route_cache<NW, kFindCacheBytes>;
+      result = tv.template read_chunk<false, rc_t, false, tree_view<NW, VER>::kFindGroup>(key, 0u, ts, nullptr, 0u,
+                                                                                           q, t, &rc);
     } else {
       uint32_t start = kNone, unused;  // each lane routes its key through the cached levels
       if (active && valid_key(key)) start = tv.route_lane(rc, key, unused);

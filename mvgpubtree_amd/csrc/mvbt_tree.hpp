@@ -135,9 +135,15 @@ struct tree_ptrs {
 template <int NW, uint32_t BYTES>
 struct route_cache {
   static constexpr uint32_t kBytes = BYTES;
-  static constexpr uint32_t kMaxNodes = kBytes / (NW * 8);
+  // Node images are NW + 1 pairs apart: lanes routing different keys read
+  // the same entry of different nodes, and an unpadded stride of NW pairs
+  // (a multiple of the LDS bank count) would put all of them on one bank.
+  static constexpr uint32_t kStride = NW + 1;
+  static constexpr uint32_t kMaxNodes = kBytes / (kStride * 8);
   static constexpr uint16_t kNoChild = 0xFFFF;
-  uint64_t nodes[kMaxNodes * NW];
+  uint64_t nodes[kMaxNodes * kStride];
+  MVBT_DEV uint64_t* img(uint32_t s) { return nodes + s * kStride; }
+  MVBT_DEV const uint64_t* img(uint32_t s) const { return nodes + s * kStride; }
   uint32_t index[kMaxNodes];        // global index of the node in slot s
   uint16_t first_child[kMaxNodes];  // slot of its first child, or kNoChild
   uint32_t count;                   // slots routed through (internal node images)
@@ -558,8 +564,11 @@ struct tree_view : tree_ptrs {
   // and the true count is returned. Keys outside `todo` return kEmpty / 0.
   // LANES: `ts` is lane j's own snapshot and `out` lane j's own output (the
   // sorted batched form's reads, each at its batch's snapshot).
+  // RG: walks advanced per pass (finds at 32- and 64-pair nodes take 4: the
+  // smaller register footprint lets two 1024-thread workgroups share a CU).
   static constexpr int kReadGroup = NW < 8 ? NW : 8;
-  template <bool RANGE, class RC = route_cache<NW, 1024>, bool LANES = false>
+  static constexpr int kFindGroup = NW < 8 ? NW : (NW >= 32 ? 4 : 8);
+  template <bool RANGE, class RC = route_cache<NW, 1024>, bool LANES = false, int RG = kReadGroup>
   MVBT_DEV uint32_t read_chunk(uint32_t lo, uint32_t hi, uint32_t ts, uint64_t* out, uint32_t cap,
                                uint64_t todo, const tile_t& t, const RC* rc = nullptr) const {
     enum : uint32_t { kRoute = 0, kArrive = 1, kChain = 2 };
@@ -572,21 +581,21 @@ struct tree_view : tree_ptrs {
       uint32_t unused;
       cur = route_lane(*rc, lo, unused);
     }
-    for (int base = 0; base < NW; base += kReadGroup) {
-      uint64_t active = (todo >> base) & ((kReadGroup == 64) ? ~0ull : ((1ull << kReadGroup) - 1));
+    for (int base = 0; base < NW; base += RG) {
+      uint64_t active = (todo >> base) & ((RG == 64) ? ~0ull : ((1ull << RG) - 1));
       for (uint32_t pass = 0; active != 0; ++pass) {
         if (pass >= kMaxReadSteps) {
           err |= kErrCorrupt;
           break;
         }
-        uint64_t p[kReadGroup];
+        uint64_t p[RG];
 #pragma unroll
-        for (int j = 0; j < kReadGroup; ++j) {
+        for (int j = 0; j < RG; ++j) {
           const uint32_t idx = t.shfl(cur, (uint32_t)(base + j));
           if ((active >> j) & 1) p[j] = ld_weak(at(idx) + r);
         }
 #pragma unroll
-        for (int j = 0; j < kReadGroup; ++j) {
+        for (int j = 0; j < RG; ++j) {
           if (!((active >> j) & 1)) continue;
           const uint32_t lj = (uint32_t)(base + j);
           node_t n;
@@ -790,7 +799,7 @@ struct tree_view : tree_ptrs {
     for (uint32_t level = 0; level < rc_t::kMaxNodes; ++level) {
       for (uint32_t w = tid; w < (e - b) * NW; w += nthreads) {
         const uint32_t s = b + w / NW;
-        rc.nodes[s * NW + w % NW] = ld_coherent(at(rc.index[s]) + w % NW);
+        rc.img(s)[w % NW] = ld_coherent(at(rc.index[s]) + w % NW);
       }
       __syncthreads();
       if (tid < 64) {  // wave 0: child counts, prefix sum, child slots
@@ -802,8 +811,8 @@ struct tree_view : tree_ptrs {
           uint32_t cnt = 0;
           bool internal = true;
           if (s < e) {  // leading entries only: an image read during a split may have gaps
-            internal = !(hi32(rc.nodes[s * NW + NW - 1]) & kLeafBit);
-            while (cnt < node_t::kData && lo32(rc.nodes[s * NW + cnt]) != kEmpty) ++cnt;
+            internal = !(hi32(rc.img(s)[NW - 1]) & kLeafBit);
+            while (cnt < node_t::kData && lo32(rc.img(s)[cnt]) != kEmpty) ++cnt;
           }
           all_internal = all_internal && __builtin_amdgcn_ballot_w64(!internal) == 0;
           uint32_t incl = cnt;  // inclusive scan over the wave
@@ -822,7 +831,7 @@ struct tree_view : tree_ptrs {
           }
           const uint32_t fc = rc.first_child[s];
           for (uint32_t l = 0; l < node_t::kData; ++l) {
-            const uint64_t pr = rc.nodes[s * NW + l];
+            const uint64_t pr = rc.img(s)[l];
             if (lo32(pr) == kEmpty) break;
             rc.index[fc + l] = hi32(pr);
           }
@@ -855,7 +864,7 @@ struct tree_view : tree_ptrs {
     parent = kNone;
     if (rc.count == 0) return root_;
     for (uint32_t step = 0; step < 2 * rc_t::kMaxNodes; ++step) {
-      const uint64_t* img = rc.nodes + s * NW;
+      const uint64_t* img = rc.img(s);
       const uint64_t m = img[node_t::kMetaLane];
       uint32_t next;
       if (key >= node_t::high_key(m)) {
@@ -1323,10 +1332,13 @@ struct tree_view : tree_ptrs {
   // version-copy slot reserved for the leaf's epoch ahead of the launch and
   // already entered in the retire ring, so no copy allocates or retires here
   // (kNone: the pool had no slot for it).
+  // The sorted form (PLANNED) takes batches of 8 leaves: its chunks hold few
+  // leaves, and the smaller batch lifts its write pass to 4 waves per SIMD.
   template <int OP, bool PLANNED = false>
   MVBT_DEV uint64_t write_chunk_at(uint32_t key, uint32_t value, uint64_t q, uint32_t leaf, uint32_t cur,
                                    uint32_t& result, write_ctx& c, const tile_t& t,
                                    uint32_t planned = kNone) const {
+    constexpr int LB = PLANNED ? 8 : kLeafBatch;
     if constexpr (NW > 32) {
       return q;  // 64-pair nodes: register budget; per-key path
     } else {
@@ -1356,19 +1368,19 @@ struct tree_view : tree_ptrs {
         uint64_t got = 0;
         {
           uint64_t m = pending;
-          for (int i = 0; i < kLeafBatch && m; ++i) {
+          for (int i = 0; i < LB && m; ++i) {
             got |= m & (~m + 1);
             m &= m - 1;
           }
         }
         pending &= ~got;
         // 2. reload this batch's leaves
-        uint64_t img[kLeafBatch];
-        uint32_t gl[kLeafBatch];
+        uint64_t img[LB];
+        uint32_t gl[LB];
         {
           uint64_t m = got;
 #pragma unroll
-          for (int i = 0; i < kLeafBatch; ++i) {
+          for (int i = 0; i < LB; ++i) {
             gl[i] = m ? (uint32_t)lowest_bit(m) : 0u;
             if (m) img[i] = ld_coherent(at(t.shfl(leaf, gl[i])) + r);
             m &= m - 1;
@@ -1386,7 +1398,7 @@ struct tree_view : tree_ptrs {
         // not all fit (or that no longer covers one of them) goes per key
         uint32_t ok = 0, changed = 0, need_copy = 0;  // bit i: slot i
 #pragma unroll
-        for (int i = 0; i < kLeafBatch; ++i) {
+        for (int i = 0; i < LB; ++i) {
           if (i >= ng) continue;
           node_t n;
           n.p = img[i];
@@ -1427,7 +1439,7 @@ struct tree_view : tree_ptrs {
         uint32_t cp = kNone;  // leader lane: copy slot of its leaf
         if constexpr (VER) {
 #pragma unroll
-          for (int i = 0; i < kLeafBatch; ++i) {
+          for (int i = 0; i < LB; ++i) {
             if (!((need_copy >> i) & 1)) continue;
             const uint32_t slot = PLANNED ? t.shfl(planned, gl[i]) : alloc(c, t, true);
             if (slot == kNone) {  // pool exhausted: leave the leaf unchanged
@@ -1451,7 +1463,7 @@ struct tree_view : tree_ptrs {
         }
         // 5. data lanes (the plan re-applied to the loaded image)
 #pragma unroll
-        for (int i = 0; i < kLeafBatch; ++i) {
+        for (int i = 0; i < LB; ++i) {
           if (!((changed >> i) & 1)) continue;
           node_t n;
           n.p = img[i];
