@@ -302,6 +302,11 @@ def run_config2(args):
         log(f"config4: batched {c4['batched_Mops']} vs plain {c4['plain_per_batch_Mops']} M ops/s")
         levels = [float(q) for q in args.extras_qps.split(",")]
         c5 = served_vs_plain(args.served_keys, levels, args.extras_requests, args.step_ops, dev, args.served_mode)
+        c3 = run_config3(args)  # erase + find at the snapshot with reclamation, same key count and steps
+        log(f"config3: {c3['Mops_incl_reclaim']} M ops/s, peak {c3['peak_live_MB']} MB")
+        result["breakdown"]["erase_find_reclaim"] = {k: v for k, v in c3.items() if k != "timeline"} | {
+            "epochs_recorded": len(c3["timeline"]) - 1,
+            "last_epoch": c3["timeline"][-1]}
         result["breakdown"]["batched_vs_plain"] = c4
         result["breakdown"]["served_vs_plain"] = {
             "keys": c5["keys"], "requests_per_level": c5["requests_per_level"],
@@ -515,7 +520,7 @@ def gpu_unique_keys(n: int, extra: int, seed: int, dev):
     return u32[:n].contiguous(), u32[n:].contiguous()
 
 
-def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev, mode: int = 2):
+def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev, mode: int = 3):
     """Served form (config 5): n-key tree resident in HBM; open-loop Poisson
     arrivals at each QPS level; request sizes log-uniform in [1, 16384] ops,
     80% find / 10% range (~32 keys) / 10% insert, one snapshot per request;
@@ -662,10 +667,10 @@ def main():
                     help="default run: skip the config-4 and config-5 legs of the breakdown")
     ap.add_argument("--extras-qps", default="8000,32000,128000,512000", help="default run: config-5 QPS levels")
     ap.add_argument("--extras-requests", type=int, default=1000, help="default run: config-5 requests per level")
-    ap.add_argument("--c4-node-width", type=int, default=32, help="default run: config-4 node width")
+    ap.add_argument("--c4-node-width", type=int, default=16, help="default run: config-4 node width (the autotuner's pick)")
     ap.add_argument("--c4-waves", type=int, default=16, help="default run: config-4 worker waves per CU")
     ap.add_argument("--c4-mode", type=int, default=2, help="default run: config-4 worker mode")
-    ap.add_argument("--served-mode", type=int, default=2, help="config 5: worker mode of the served steps")
+    ap.add_argument("--served-mode", type=int, default=3, help="config 5: worker mode of the served steps (3: by step size)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()

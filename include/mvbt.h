@@ -275,8 +275,9 @@ typedef struct mvbt_op {
  * concurrent_insert_range / concurrent_find_erase (gpu_versioned_blink_tree.hpp:218-294,
  * btree_kernels.hpp:31-306) to any mix, deterministically. Runs num_batches batches in
  * one launch sequence (a persistent work-queue worker in worker modes 0 and 1; the
- * sorted form in mode 2, the default: every write grouped by leaf and applied in batch
- * order, then every read at its batch's snapshot). batch_offsets (device, num_batches + 1 entries) are
+ * sorted form in mode 2: every write grouped by leaf and applied in batch order, then
+ * every read at its batch's snapshot; mode 3, the default, picks by call size, see
+ * mvbt_set_worker_mode). batch_offsets (device, num_batches + 1 entries) are
  * absolute indices into ops: batch b is ops[offsets[b] .. offsets[b+1]), results go to
  * the same indices (so `results` must cover index offsets[num_batches] - 1), and
  * offsets[num_batches] - offsets[0] must equal num_ops (so a launch can cover any slice
@@ -312,8 +313,9 @@ mvbt_status mvbt_run_batches(mvbt_tree* tree, const mvbt_op* ops, uint64_t num_o
 
 /* No counterpart in the reference (a capacity hint, like std::vector::reserve): sizes
  * the batched form's scratch for calls of up to num_ops operations and num_batches
- * batches and builds the launch graph of the current worker mode, so that later such
- * calls allocate nothing and build nothing. Synchronous. */
+ * batches and builds the launch graphs of the current worker mode (in mode 3, of both
+ * forms a call of that size or smaller may run in), so that later such calls allocate
+ * nothing and build nothing. Synchronous. */
 mvbt_status mvbt_reserve_batches(mvbt_tree* tree, uint64_t num_ops, uint32_t num_batches);
 
 /* No counterpart in the reference. For callers that cannot allocate device memory
@@ -346,10 +348,12 @@ mvbt_status mvbt_set_worker_waves(mvbt_tree* tree, uint32_t waves_per_cu);
 /* Work unit of the batched form (autotuner knob): 0 = one operation per tile of a
  * persistent worker that opens batch after batch; 1 = the same worker with runs of
  * consecutive operations per tile, handled with the chunk routing and batched leaf
- * writes of the per-batch kernels; 2 = the sorted form (default): the call's writes
- * are routed to their leaves, sorted by leaf (stable radix sort), and applied leaf group
- * by leaf group in batch order by the tile that owns the group, then every find and
- * range runs at its batch's snapshot in one pass. Same results in every mode. */
+ * writes of the per-batch kernels; 2 = the sorted form: the call's writes are routed
+ * to their leaves, sorted by leaf (stable radix sort), and applied leaf group by leaf
+ * group in batch order by the tile that owns the group, then every find and range runs
+ * at its batch's snapshot in one pass; 3 (default) = by call size: mode 0 for calls of
+ * fewer than 16384 operations (the sorted form's launch sequence then costs more than
+ * it saves), mode 2 from there up. Same results in every mode. */
 mvbt_status mvbt_set_worker_mode(mvbt_tree* tree, uint32_t mode);
 
 /* Operations per window of the sorted form (autotuner knob; default 2^22; 0 = the whole

@@ -462,6 +462,7 @@ class GpuVersionedBTree(_TreeBase):
 
     def set_worker_mode(self, mode: int):
         """0: one operation per tile; 1: runs of operations per tile; 2: the
-        sorted form (default): writes grouped by leaf in batch order, then
-        every read at its batch's snapshot."""
+        sorted form: writes grouped by leaf in batch order, then every read at
+        its batch's snapshot; 3 (default): mode 0 for calls of fewer than
+        16384 operations, mode 2 from there up."""
         _check(load_library().mvbt_set_worker_mode(self._h, mode), "set_worker_mode")

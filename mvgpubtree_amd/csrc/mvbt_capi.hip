@@ -109,8 +109,11 @@ struct mvbt_tree {
   uint64_t b_sorted_cap = 0;  // entries per array
   uint32_t worker_waves = 16;
   // 0: one operation per tile; 1: chunks (batch_chunk_worker_kernel);
-  // 2: sorted (writes grouped by leaf, then every read at its snapshot)
-  uint32_t worker_mode = 2;
+  // 2: sorted (writes grouped by leaf, then every read at its snapshot);
+  // 3: by call size (the persistent worker below kAutoSortedOps operations,
+  // the sorted form from there up). eff_mode: the mode of the current call.
+  uint32_t worker_mode = 3;
+  uint32_t eff_mode = 2;
   // launch graphs: [0] for worker modes 0 and 1, [k] for the sorted form in k windows
   static constexpr uint32_t kMaxWindows = 16;
   hipGraph_t graph[kMaxWindows + 1] = {};
@@ -585,7 +588,7 @@ mvbt_status ensure_batch_scratch(mvbt_tree* t, uint32_t nb, uint64_t nops) {
     MVBT_HIP(hipMemset(t->job, 0, sizeof(batch_job)));
   }
   const uint64_t want_hash = conflict_entries(nops);
-  const bool sorted = t->worker_mode == 2;
+  const bool sorted = t->eff_mode == 2;
   if (nb <= t->b_capacity && want_hash <= t->b_hash_cap && nops <= t->b_op_cap &&
       (!sorted || nops <= t->b_sorted_cap))
     return MVBT_OK;
@@ -783,7 +786,7 @@ mvbt_status sorted_sequence(mvbt_tree* t, Add&& add, batch_job* job, uint32_t* a
 
 void* worker_kernel(const mvbt_tree* t) {
   const dispatch_table* tb = table_for(t->nw, true);
-  return t->worker_mode == 1 ? tb->chunk_worker() : tb->worker();
+  return t->eff_mode == 1 ? tb->chunk_worker() : tb->worker();
 }
 
 void destroy_graph(mvbt_tree* t) {
@@ -799,11 +802,20 @@ void destroy_graph(mvbt_tree* t) {
 // Windows of a sorted-form call: about window_ops operations each, at most
 // one per batch and kMaxWindows.
 uint32_t windows_for(const mvbt_tree* t, uint64_t num_ops, uint32_t num_batches) {
-  if (t->worker_mode != 2 || t->window_ops == 0) return 1;
+  if (t->eff_mode != 2 || t->window_ops == 0) return 1;
   const uint64_t k = (num_ops + t->window_ops - 1) / t->window_ops;
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({k, num_batches, mvbt_tree::kMaxWindows}));
 }
-uint32_t graph_index(const mvbt_tree* t, uint32_t windows) { return t->worker_mode == 2 ? windows : 0; }
+uint32_t graph_index(const mvbt_tree* t, uint32_t windows) { return t->eff_mode == 2 ? windows : 0; }
+
+// Below this many operations a call in mode 3 runs on the one-operation-per-
+// tile worker: the sorted form's 25-odd launches per window then cost more
+// than the frontier waits they save (served steps of one or two requests).
+constexpr uint64_t kAutoSortedOps = 16384;
+uint32_t effective_mode(const mvbt_tree* t, uint64_t num_ops) {
+  if (t->worker_mode != 3) return t->worker_mode;
+  return num_ops < kAutoSortedOps ? 0u : 2u;
+}
 
 // Explicit graph: setup -> count -> mask -> clear -> conflict -> (frontier -> worker | the sorted
 // form's windows) -> finish. Every node but setup reads its inputs from the
@@ -836,7 +848,7 @@ mvbt_status build_graph(mvbt_tree* t, const batch_params& bp) {
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_mask_kernel), 1, 64, job_args));
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_clear_kernel), t->num_cus * 4, kBlock, job_args));
   MVBT_HIP(add(reinterpret_cast<void*>(&batch_conflict_kernel), t->num_cus * 8, kBlock, count_args));
-  if (t->worker_mode == 2) {
+  if (t->eff_mode == 2) {
     if (mvbt_status rc = sorted_sequence(t, add, t->job, t->b_sorted, t->b_sorted_cap, bp.windows)) return rc;
   } else {
     MVBT_HIP(add(reinterpret_cast<void*>(&batch_frontier_kernel), t->num_cus, kBlock, job_args));
@@ -1271,6 +1283,7 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
   if (pairs && L == 0) return fail(MVBT_ERR_INVALID_ARG, "L must be > 0 when pairs is given");
   device_guard g(t->device);
   std::lock_guard<std::mutex> lk(t->mu);
+  t->eff_mode = effective_mode(t, num_ops);
   if (mvbt_status rc = reserve_ids(t, num_batches)) return rc;
   if (mvbt_status rc = ensure_batch_scratch(t, num_batches, num_ops)) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -1310,7 +1323,7 @@ mvbt_status mvbt_run_batches(mvbt_tree* t, const mvbt_op* ops, uint64_t num_ops,
     hipLaunchKernelGGL(batch_mask_kernel, dim3(1), dim3(64), 0, s, t->job);
     hipLaunchKernelGGL(batch_clear_kernel, dim3(t->num_cus * 4), dim3(kBlock), 0, s, t->job);
     hipLaunchKernelGGL(batch_conflict_kernel, dim3(t->num_cus * 8), dim3(kBlock), 0, s, t->job, t->st);
-    if (t->worker_mode == 2) {
+    if (t->eff_mode == 2) {
       auto launch = [s](void* fn, unsigned grid, unsigned block, void** args) -> hipError_t {
         return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
       };
@@ -1338,15 +1351,21 @@ mvbt_status mvbt_reserve_batches(mvbt_tree* t, uint64_t num_ops, uint32_t num_ba
   if (num_ops >= (1ull << 31)) return fail(MVBT_ERR_INVALID_ARG, "num_ops must be below 2^31");
   device_guard g(t->device);
   std::lock_guard<std::mutex> lk(t->mu);
-  if (mvbt_status rc = ensure_batch_scratch(t, std::max<uint32_t>(num_batches, 1), std::max<uint64_t>(num_ops, 1)))
-    return rc;
-  const uint32_t kmax = windows_for(t, num_ops, std::max<uint32_t>(num_batches, 1));
-  for (uint32_t k = 1; k <= kmax; ++k) {
-    batch_params p{};
-    p.hash_mask = 1023;
-    p.windows = k;
-    if (!t->graph_exec[graph_index(t, k)])
-      if (mvbt_status rc = build_graph(t, p)) return rc;
+  // every mode calls of up to num_ops operations may run in (both, in mode 3)
+  std::vector<uint32_t> modes{effective_mode(t, num_ops)};
+  if (t->worker_mode == 3 && num_ops >= kAutoSortedOps) modes.push_back(0);
+  for (uint32_t m : modes) {
+    t->eff_mode = m;
+    if (mvbt_status rc = ensure_batch_scratch(t, std::max<uint32_t>(num_batches, 1), std::max<uint64_t>(num_ops, 1)))
+      return rc;
+    const uint32_t kmax = windows_for(t, num_ops, std::max<uint32_t>(num_batches, 1));
+    for (uint32_t k = 1; k <= kmax; ++k) {
+      batch_params p{};
+      p.hash_mask = 1023;
+      p.windows = k;
+      if (!t->graph_exec[graph_index(t, k)])
+        if (mvbt_status rc = build_graph(t, p)) return rc;
+    }
   }
   MVBT_HIP(hipDeviceSynchronize());
   return MVBT_OK;
@@ -1446,8 +1465,9 @@ mvbt_status mvbt_set_worker_waves(mvbt_tree* t, uint32_t waves_per_cu) {
 
 mvbt_status mvbt_set_worker_mode(mvbt_tree* t, uint32_t mode) {
   if (mvbt_status rc = check_tree(t)) return rc;
-  if (mode > 2)
-    return fail(MVBT_ERR_INVALID_ARG, "worker mode is 0 (one operation per tile), 1 (chunks) or 2 (sorted)");
+  if (mode > 3)
+    return fail(MVBT_ERR_INVALID_ARG,
+                "worker mode is 0 (one operation per tile), 1 (chunks), 2 (sorted) or 3 (by call size)");
   std::lock_guard<std::mutex> lk(t->mu);
   if (t->worker_mode != mode) {
     device_guard g(t->device);

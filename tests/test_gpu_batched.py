@@ -298,3 +298,55 @@ def test_sorted_form_fits_a_pool_that_holds_one_window_of_copies(dev):
     assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
     assert tree.validate_tree_structure()[0] == final.keys.size
     tree.close()
+
+
+def test_no_reclaim_flag_keeps_every_copy(dev):
+    """MVBT_NO_RECLAIM: a call in many windows frees nothing between them
+    (the timeline gains no epoch); a later mvbt_reclaim frees the copies."""
+    w = small_config4(seed=142, num_batches=120)
+    tree = build_tree(w, dev, 16)
+    tree.set_worker_mode(2)
+    tree.set_batch_window(10_000)
+    e0 = len(tree.memory_series())
+    r = drivers.BatchedRunner(w.batches, w.range_capacity, dev, seed=7)
+    tree.run_batches(r.ops, r.offsets, results=r.results, pairs=r.pairs, pair_slots=max(r.slots, 1),
+                     average_range_length=w.range_capacity, use_graph=True, no_reclaim=True)
+    want, wpairs, _ = oracle.batch_expected(w.initial_keys, w.initial_values, r.ops_host, r.offsets_host,
+                                            w.range_capacity, r.slots)
+    assert np.array_equal(to_np(r.results)[: r.nops], want)
+    assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
+    assert len(tree.memory_series()) == e0
+    pending = tree.memory_stats().retired_pending
+    assert pending > 0
+    tree.reclaim()
+    ser = tree.memory_series()
+    assert len(ser) == e0 + 1 and int(ser.freed_nodes[-1]) == pending
+    assert tree.memory_stats().retired_pending == 0
+    assert tree.error_flags() == 0
+    tree.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_auto_mode_switches_by_call_size(dev, use_graph):
+    """Mode 3 (default): calls below 16384 operations run on the persistent
+    worker, larger ones in the sorted form; a run of small, large, small
+    calls on one tree (graphs of both forms held at once) stays bit-exact."""
+    w = small_config4(seed=143, num_batches=60, max_batch=2048)
+    tree = build_tree(w, dev, 32)
+    tree.reserve_batches(40_000, 64)
+    keys, vals = w.initial_keys, w.initial_values
+    cuts = [0, 4, 50, 60]  # batches of each call
+    for i, (a, b) in enumerate(zip(cuts[:-1], cuts[1:])):
+        r = drivers.BatchedRunner(w.batches[a:b], w.range_capacity, dev, seed=20 + i)
+        assert (r.nops < 16384) == (i != 1), r.nops
+        res = to_np(r.run(tree, use_graph=use_graph))
+        want, wpairs, final = oracle.batch_expected(keys, vals, r.ops_host, r.offsets_host, w.range_capacity,
+                                                    r.slots)
+        bad = np.nonzero(res != want)[0]
+        assert bad.size == 0, (i, bad[:10], r.ops_host[bad[:10]], res[bad[:10]], want[bad[:10]])
+        assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
+        keys, vals = final.keys, final.values
+    exp = tree.export()
+    assert np.array_equal(exp, keys.astype(np.uint64) | (vals.astype(np.uint64) << np.uint64(32)))
+    assert tree.error_flags() == 0
+    tree.close()

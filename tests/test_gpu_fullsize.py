@@ -180,7 +180,7 @@ def test_config5_full_size(dev):
     ks = ks_dev.cpu().numpy().view(np.uint32)
     note(f"config5: {n} keys generated ({time.time() - t0:.1f}s)")
     tree = mv.GpuVersionedBTree(pool_nodes=int(n / 6) + (8 << 20))
-    tree.set_worker_mode(2)
+    tree.set_worker_mode(3)  # by step size: small steps on the persistent worker, large ones sorted
     order = torch.randperm(n, generator=g, device=dev)
     k_ins = ks_dev[order]
     del order
