@@ -155,8 +155,9 @@ def run_config2(args):
 
     def make_tree(n):
         pool = int((n + args.steps * args.ops) / 4) + (1 << 20)
-        return mv.GpuVersionedBTree(pool_nodes=pool, node_width=args.node_width,
-                                    retire_capacity=1 << 23)
+        t = mv.GpuVersionedBTree(pool_nodes=pool, node_width=args.node_width, retire_capacity=1 << 23)
+        t.set_read_tile(args.read_tile if args.read_tile < args.node_width else 0)
+        return t
 
     scratch = make_tree(ww.initial_keys.size)
     scratch.insert(dev_tensor(ww.initial_keys, dev), dev_tensor(ww.initial_values, dev))
@@ -229,6 +230,7 @@ def run_config2(args):
         ftree = mv.GpuVersionedBTree(pool_nodes=int(n_initial / 4) + (1 << 20), node_width=args.find_node_width,
                                      retire_capacity=1 << 20)
         ftree.insert(dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev), stream=stream)
+        ftree.set_read_tile(args.read_tile if args.read_tile < args.find_node_width else 0)
     sf = ftree.take_snapshot(stream)
     for _ in range(2):
         ftree.find(dq, fout, timestamp=sf, stream=stream)
@@ -281,6 +283,7 @@ def run_config2(args):
                                "new keys + 50% range queries (~32 keys, L=128) at the pre-batch "
                                "snapshot; plain per-batch launch path",
                    "initial_keys": n_initial, "ops_per_batch": args.ops, "node_width": args.node_width,
+                   "read_tile": min(args.read_tile or args.node_width, args.node_width),
                    "range_span": w.span, "range_capacity": L},
         "roofline": roofline,
         "breakdown": {k: round(v, 2) for k, v in rates.items()} | {
@@ -297,7 +300,8 @@ def run_config2(args):
         # (batched form vs plain per-batch launches) and config 5 (served form
         # vs plain per-request launches, p50/p99 latency) at full size
         w4 = wl.config4(seed=4, n_initial=n_initial, num_batches=4096)
-        c4 = batched_vs_plain(w4, w4.range_capacity, dev, args.c4_node_width, args.c4_mode, args.c4_waves)
+        c4 = batched_vs_plain(w4, w4.range_capacity, dev, args.c4_node_width, args.c4_mode, args.c4_waves,
+                              read_tile=args.c4_read_tile)
         del w4
         log(f"config4: batched {c4['batched_Mops']} vs plain {c4['plain_per_batch_Mops']} M ops/s")
         levels = [float(q) for q in args.extras_qps.split(",")]
@@ -385,7 +389,7 @@ def run_config3(args):
             "reclaimed_MB": round(st.reclaimed_nodes * st.node_bytes / 1e6, 1), "timeline": timeline}
 
 
-def batched_vs_plain(w, L, dev, nw: int, mode: int, waves: int, plain: bool = True):
+def batched_vs_plain(w, L, dev, nw: int, mode: int, waves: int, plain: bool = True, read_tile: int = 0):
     """Config 4 on a fresh tree per path: the whole stream in one batched
     call through the captured hipGraph, against the plain per-batch launch
     path (C++ loop over the C ABI: snapshot, find, range, insert, erase,
@@ -409,6 +413,7 @@ def batched_vs_plain(w, L, dev, nw: int, mode: int, waves: int, plain: bool = Tr
         t.insert(dk, dv)
         t.set_worker_waves(waves)
         t.set_worker_mode(mode)
+        t.set_read_tile(read_tile)
         return t
 
     # warm both paths (code load, graph build) on a small scratch tree
@@ -417,6 +422,7 @@ def batched_vs_plain(w, L, dev, nw: int, mode: int, waves: int, plain: bool = Tr
     scratch.insert(dev_tensor(wb.initial_keys, dev), dev_tensor(wb.initial_values, dev))
     scratch.set_worker_waves(waves)
     scratch.set_worker_mode(mode)
+    scratch.set_read_tile(read_tile)
     drivers.BatchedRunner(wb.batches, L, dev).run(scratch, use_graph=True)
     drivers.PerBatchRunner(wb.batches[:8], L, dev).run(scratch)
     torch.cuda.synchronize()
@@ -459,7 +465,7 @@ def batched_vs_plain(w, L, dev, nw: int, mode: int, waves: int, plain: bool = Tr
         "batched_ms": round(batched_s * 1e3, 2), "plain_ms": round(plain_s * 1e3, 2) if plain else None,
         "ops": n_ops, "batches": len(w.batches), "initial_keys": int(n0),
         "failed_writes": failed,
-        "node_width": nw, "worker_mode": mode, "waves_per_cu": waves,
+        "node_width": nw, "read_tile": read_tile or nw, "worker_mode": mode, "waves_per_cu": waves,
     }
 
 
@@ -483,20 +489,23 @@ def run_config4(args):
     waves = tuple(int(x) for x in args.waves.split(","))
     widths = tuple(int(x) for x in args.widths.split(","))
     modes = tuple(int(x) for x in args.modes.split(","))
-    ncfg = len(waves) * len(modes)
+    tiles = tuple(int(x) for x in args.tiles.split(","))
+    ncfg = len(waves) * len(modes) * len(tiles)
     if len(widths) * ncfg > 1:
         tw = wl.config4(seed=44, n_initial=args.keys, num_batches=2 * args.tune_batches)
         groups = [tw.batches[0::2], tw.batches[1::2]]  # (warm, timed), the same for every configuration
         best, allr = autotune.tune_batched(tw.initial_keys, tw.initial_values, groups, L, dev,
-                                           node_widths=widths, waves=waves, log=log, modes=modes)
+                                           node_widths=widths, waves=waves, log=log, modes=modes, tiles=tiles)
         del tw
     else:
-        best, allr = autotune.TuneResult(widths[0], waves[0], 0.0, 0.0, modes[0]), []
-    log(f"autotune best: node width {best.node_width}, worker mode {best.worker_mode}, "
-        f"{best.waves_per_cu} waves/CU")
-    out = batched_vs_plain(w, L, dev, best.node_width, best.worker_mode, best.waves_per_cu, plain=not args.no_plain)
-    out["tuning"] = [{"nw": r.node_width, "mode": r.worker_mode, "waves": r.waves_per_cu, "Mops": round(r.mops, 1)}
-                     for r in allr]
+        best, allr = autotune.TuneResult(widths[0], waves[0], 0.0, 0.0, modes[0],
+                                         tiles[0] if tiles[0] < widths[0] else 0), []
+    log(f"autotune best: node width {best.node_width}, read tile {best.read_tile or best.node_width} lanes, "
+        f"worker mode {best.worker_mode}, {best.waves_per_cu} waves/CU")
+    out = batched_vs_plain(w, L, dev, best.node_width, best.worker_mode, best.waves_per_cu, plain=not args.no_plain,
+                           read_tile=best.read_tile)
+    out["tuning"] = [{"nw": r.node_width, "tile": r.read_tile or r.node_width, "mode": r.worker_mode,
+                      "waves": r.waves_per_cu, "Mops": round(r.mops, 1)} for r in allr]
     return out
 
 
@@ -643,6 +652,9 @@ def main():
     ap.add_argument("--node-width", type=int, default=32,
                     help="config 2: node width of the measured tree (32 pairs measured fastest for "
                          "insert + range-32 batches)")
+    ap.add_argument("--read-tile", type=int, default=16,
+                    help="config 2: lanes per read tile of the range and find launches (0: the node width; "
+                         "16-lane tiles over 32-pair nodes measured fastest)")
     ap.add_argument("--find-node-width", type=int, default=32,
                     help="config 2: node width of the tree the find breakdown runs on (default: the "
                          "measured tree itself)")
@@ -661,15 +673,19 @@ def main():
     ap.add_argument("--modes", default="1,2",
                     help="config 4: worker modes to tune (0 one op per tile, 1 runs, 2 sorted)")
     ap.add_argument("--widths", default="16,32,64", help="config 4: node widths to tune")
+    ap.add_argument("--tiles", default="16,32,64",
+                    help="config 4: read tile widths (lanes) to tune, each at most the node width")
     ap.add_argument("--no-plain", action="store_true", help="config 4: batched form only (profiling)")
     ap.add_argument("--waves", default="8,16,24,32", help="config 4: waves/CU to tune")
     ap.add_argument("--no-extras", action="store_true",
                     help="default run: skip the config-4 and config-5 legs of the breakdown")
     ap.add_argument("--extras-qps", default="8000,32000,128000,512000", help="default run: config-5 QPS levels")
     ap.add_argument("--extras-requests", type=int, default=1000, help="default run: config-5 requests per level")
-    ap.add_argument("--c4-node-width", type=int, default=16, help="default run: config-4 node width (the autotuner's pick)")
+    ap.add_argument("--c4-node-width", type=int, default=32, help="default run: config-4 node width (the autotuner's pick)")
     ap.add_argument("--c4-waves", type=int, default=16, help="default run: config-4 worker waves per CU")
     ap.add_argument("--c4-mode", type=int, default=2, help="default run: config-4 worker mode")
+    ap.add_argument("--c4-read-tile", type=int, default=16,
+                    help="default run: config-4 read tile lanes (0: the node width; the autotuner's pick)")
     ap.add_argument("--served-mode", type=int, default=3, help="config 5: worker mode of the served steps (3: by step size)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")

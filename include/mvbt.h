@@ -356,6 +356,15 @@ mvbt_status mvbt_set_worker_waves(mvbt_tree* tree, uint32_t waves_per_cu);
  * it saves), mode 2 from there up. Same results in every mode. */
 mvbt_status mvbt_set_worker_mode(mvbt_tree* tree, uint32_t mode);
 
+/* Read tile width (autotuner knob; the reference's tile is always its node width,
+ * cg::tiled_partition<branching_factor> in btree_kernels.hpp): lanes per tile of the
+ * launches in which no writer runs, i.e. mvbt_find and mvbt_range_query without
+ * MVBT_CONCURRENT and the sorted form's read passes. 0 or the node width: one pair per
+ * lane (default); 16 or 32 below the node width: each lane reads node_width / lanes
+ * (2 or 4) consecutive pairs, and a wave walks 64 / lanes keys at once. Writers and
+ * concurrent reads keep one pair per lane. Same results for every width. */
+mvbt_status mvbt_set_read_tile(mvbt_tree* tree, uint32_t lanes);
+
 /* Operations per window of the sorted form (autotuner knob; default 2^22; 0 = the whole
  * call in one window). Smaller windows free copies sooner and shorten the version chains
  * that reads walk; each window costs about 20 launches. No reference counterpart. */

@@ -55,7 +55,7 @@ EXPORTED_SYMBOLS = (
     "mvbt_memory_stats", "mvbt_error_flags", "mvbt_clear_error_flags", "mvbt_validate",
     "mvbt_export", "mvbt_node_width", "mvbt_kind_of", "mvbt_run_batches", "mvbt_set_worker_waves",
     "mvbt_memory_stats_async", "mvbt_device_view", "mvbt_bulk_build", "mvbt_concurrent_insert_range",
-    "mvbt_concurrent_find_erase", "mvbt_export_dot", "mvbt_set_worker_mode", "mvbt_run_batches_noalloc",
+    "mvbt_concurrent_find_erase", "mvbt_export_dot", "mvbt_set_worker_mode", "mvbt_set_read_tile", "mvbt_run_batches_noalloc",
     "mvbt_reserve_batches", "mvbt_memory_series", "mvbt_set_batch_window",
 )
 
@@ -150,6 +150,7 @@ def load_library():
         "mvbt_set_batch_window": ([vp, u64], i32),
         "mvbt_set_worker_waves": ([vp, u32], i32),
         "mvbt_set_worker_mode": ([vp, u32], i32),
+        "mvbt_set_read_tile": ([vp, u32], i32),
         "mvbt_memory_stats_async": ([vp, vp, vp], i32),
         "mvbt_device_view": ([vp, vp], i32),
         "mvbt_bulk_build": ([vp, vp, vp, u64, u32, vp], i32),
@@ -361,6 +362,13 @@ class _TreeBase:
         host memory: a torch int64 tensor of 8 elements)."""
         _check(load_library().mvbt_memory_stats_async(self._h, _ptr_any(out, 8, "out"), _stream(stream)),
                "memory_stats_async")
+
+    def set_read_tile(self, lanes: int):
+        """Lanes per tile of the launches in which no writer runs (finds and
+        ranges without `concurrent`, the sorted form's read passes): 0 or the
+        node width (one pair per lane), or 16 / 32 below it (node_width /
+        lanes pairs per lane)."""
+        _check(load_library().mvbt_set_read_tile(self._h, lanes), "set_read_tile")
 
     def error_flags(self) -> int:
         f = ctypes.c_uint32()

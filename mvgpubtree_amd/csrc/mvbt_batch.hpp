@@ -1059,14 +1059,15 @@ __global__ void __launch_bounds__(256) batch_sorted_write_kernel(tree_ptrs tp, b
   tv.finish(ctx, t);
 }
 
-// 9. Finds, each at its batch's snapshot (no writer runs).
-template <int NW>
+// 9. Finds, each at its batch's snapshot (no writer runs). T: read tile
+// width (T < NW: P = NW / T pairs per lane, mvbt_wide.hpp).
+template <int NW, int T = NW>
 __global__ void __launch_bounds__(kFindBlock) __attribute__((amdgpu_waves_per_eu(8)))
 batch_sorted_find_kernel(tree_ptrs tp, const batch_job* job) {
   if (ld32_coherent(&job->bad)) return;
   const tree_view<NW, true> tv(tp);
-  tile<NW> t;
-  chunking<NW> c;
+  tile<T> t;
+  chunking<T> c;
   __shared__ route_cache<NW, kFindCacheBytes> rc;
   tv.build_route_cache(rc);
   const uint32_t n = job->n_finds;
@@ -1082,7 +1083,10 @@ batch_sorted_find_kernel(tree_ptrs tp, const batch_job* job) {
     const uint32_t key = act ? ops[rel].key : 0u;
     const uint32_t ts = act ? clock0 + op_batch[rel] : 0u;
     uint32_t result = kEmpty;
-    if constexpr (NW >= 32) {
+    if constexpr (T < NW) {
+      result = read_chunk_wide<T, false, true, kWideFindGroup<NW / T>>(tv, key, 0u, ts, nullptr, 0u, t.ballot(act), t,
+                                                                        &rc);
+    } else if constexpr (NW >= 32) {
       // as in find_kernel: the pass-based engine keeps more walks in flight
       result = tv.template read_chunk<false, route_cache<NW, kFindCacheBytes>, true, tree_view<NW, true>::kFindGroup>(
           key, 0u, ts, nullptr, 0u, t.ballot(act), t, &rc);
@@ -1101,14 +1105,14 @@ batch_sorted_find_kernel(tree_ptrs tp, const batch_job* job) {
 }
 
 // 10. Ranges, each at its batch's snapshot: the pass-based reader with a
-// snapshot and an output slot per lane.
-template <int NW>
+// snapshot and an output slot per lane (T: read tile width, as above).
+template <int NW, int T = NW>
 __global__ void __launch_bounds__(256) batch_sorted_range_kernel(tree_ptrs tp, const batch_job* job) {
   if (ld32_coherent(&job->bad)) return;
   using rc_t = route_cache<NW, kRangeCacheBytes>;
   const tree_view<NW, true> tv(tp);
-  tile<NW> t;
-  chunking<NW> c;
+  tile<T> t;
+  chunking<T> c;
   __shared__ rc_t rc;
   tv.build_route_cache(rc);
   const uint32_t n = job->n_ranges, L = job->L;
@@ -1132,13 +1136,18 @@ __global__ void __launch_bounds__(256) batch_sorted_range_kernel(tree_ptrs tp, c
       if (op.slot < pair_slots) out = pairs + (size_t)op.slot * L;
       else bad_slot = true;
     }
-    const uint32_t cnt = tv.template read_chunk<true, rc_t, true>(op.key == 0 ? 1u : op.key, op.arg, ts, out, L,
-                                                                  t.ballot(act), t, &rc);
+    uint32_t cnt;
+    if constexpr (T < NW)
+      cnt = read_chunk_wide<T, true, true, kWideRangeGroup<NW / T>>(tv, op.key == 0 ? 1u : op.key, op.arg, ts, out, L,
+                                                                    t.ballot(act), t, &rc);
+    else
+      cnt = tv.template read_chunk<true, rc_t, true>(op.key == 0 ? 1u : op.key, op.arg, ts, out, L, t.ballot(act), t,
+                                                     &rc);
     if (act) results[rel] = cnt;
     overflow |= act && cnt > L;
   }
-  if (t.any(overflow)) tv.flag(kErrRangeOverflow, t);
-  if (t.any(bad_slot)) tv.flag(kErrCorrupt, t);
+  const uint32_t err = (t.any(overflow) ? kErrRangeOverflow : 0u) | (t.any(bad_slot) ? kErrCorrupt : 0u);
+  if (err && t.leader()) atomic_or32(&tv.st->error_flags, err);
 }
 
 }  // namespace mvbt

@@ -120,6 +120,10 @@ struct mvbt_tree {
   hipGraphExec_t graph_exec[kMaxWindows + 1] = {};
   hipGraphNode_t graph_setup[kMaxWindows + 1] = {};
   uint64_t window_ops = 1ull << 22;  // sorted form: operations per window (0: one window)
+  // lanes per read tile of the launches in which no writer runs (finds and
+  // ranges without MVBT_CONCURRENT, the sorted form's read passes); below the
+  // node width each lane reads NW / read_tile pairs
+  uint32_t read_tile = 0;  // 0: the node width
   bool external_writers = false;  // a device view was handed out
   bool fresh = true;              // no write, snapshot or device view yet (bulk build allowed)
 
@@ -172,9 +176,38 @@ struct impl {
     MVBT_HIP(hipGetLastError());
     return MVBT_OK;
   }
+  // Read launches with tiles narrower than the node (no writer runs).
+  template <int T>
+  static bool find_wide(mvbt_tree* t, const uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t ts, hipStream_t s,
+                        uint32_t g) {
+    if constexpr (T < NW && NW / T <= 4) {
+      if (t->read_tile == (uint32_t)T) {
+        hipLaunchKernelGGL((find_wide_kernel<NW, VER, T>), dim3(g), dim3(kFindBlock), 0, s,
+                           t->template view<NW, VER>(), keys, vals, n, ts);
+        return true;
+      }
+    }
+    return false;
+  }
+  template <int T>
+  static bool range_wide(mvbt_tree* t, const uint32_t* lo, const uint32_t* hi, uint64_t* out, uint32_t* counts,
+                         uint32_t L, uint64_t n, uint32_t ts, hipStream_t s, uint32_t g) {
+    if constexpr (T < NW && NW / T <= 4) {
+      if (t->read_tile == (uint32_t)T) {
+        hipLaunchKernelGGL((range_wide_kernel<NW, VER, T>), dim3(g), dim3(kBlock), 0, s,
+                           t->template view<NW, VER>(), lo, hi, out, counts, L, n, ts);
+        return true;
+      }
+    }
+    return false;
+  }
   static mvbt_status find(mvbt_tree* t, const uint32_t* keys, uint32_t* vals, uint64_t n,
                           uint32_t ts, hipStream_t s, bool conc) {
     uint32_t g = grid_for(t, n, NW, 32, kFindBlock);
+    if (!conc && (find_wide<16>(t, keys, vals, n, ts, s, g) || find_wide<32>(t, keys, vals, n, ts, s, g))) {
+      MVBT_HIP(hipGetLastError());
+      return MVBT_OK;
+    }
     if (conc)
       hipLaunchKernelGGL((find_kernel<NW, VER, true>), dim3(g), dim3(kFindBlock), 0, s,
                          t->template view<NW, VER>(), keys, vals, n, ts);
@@ -188,6 +221,11 @@ struct impl {
                            uint32_t* counts, uint32_t L, uint64_t n, uint32_t ts, hipStream_t s,
                            bool conc) {
     uint32_t g = grid_for(t, n, NW, 32);
+    if (!conc && (range_wide<16>(t, lo, hi, out, counts, L, n, ts, s, g) ||
+                  range_wide<32>(t, lo, hi, out, counts, L, n, ts, s, g))) {
+      MVBT_HIP(hipGetLastError());
+      return MVBT_OK;
+    }
     if (conc)
       hipLaunchKernelGGL((range_kernel<NW, VER, true>), dim3(g), dim3(kBlock), 0, s,
                          t->template view<NW, VER>(), lo, hi, out, counts, L, n, ts);
@@ -242,11 +280,24 @@ struct sorted_fns {
   void* range;
 };
 template <int NW>
-sorted_fns sorted_fn() {
-  return sorted_fns{reinterpret_cast<void*>(&batch_route_kernel<NW>),
-                    reinterpret_cast<void*>(&batch_sorted_write_kernel<NW>),
-                    reinterpret_cast<void*>(&batch_sorted_find_kernel<NW>),
-                    reinterpret_cast<void*>(&batch_sorted_range_kernel<NW>)};
+sorted_fns sorted_fn(uint32_t read_tile) {
+  sorted_fns f{reinterpret_cast<void*>(&batch_route_kernel<NW>),
+               reinterpret_cast<void*>(&batch_sorted_write_kernel<NW>),
+               reinterpret_cast<void*>(&batch_sorted_find_kernel<NW>),
+               reinterpret_cast<void*>(&batch_sorted_range_kernel<NW>)};
+  if constexpr (NW >= 32) {
+    if (read_tile == 16) {
+      f.find = reinterpret_cast<void*>(&batch_sorted_find_kernel<NW, 16>);
+      f.range = reinterpret_cast<void*>(&batch_sorted_range_kernel<NW, 16>);
+    }
+  }
+  if constexpr (NW == 64) {
+    if (read_tile == 32) {
+      f.find = reinterpret_cast<void*>(&batch_sorted_find_kernel<NW, 32>);
+      f.range = reinterpret_cast<void*>(&batch_sorted_range_kernel<NW, 32>);
+    }
+  }
+  return f;
 }
 template <int NW>
 void* chunk_worker_fn() {
@@ -262,7 +313,7 @@ struct dispatch_table {
   mvbt_status (*erase)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, hipStream_t);
   void* (*worker)();
   void* (*chunk_worker)();
-  sorted_fns (*sorted)();
+  sorted_fns (*sorted)(uint32_t read_tile);
   mvbt_status (*bulk)(mvbt_tree*, const uint32_t*, const uint32_t*, const bulk_plan&, hipStream_t);
   uint32_t data_lanes;
   using mixed_fn = mvbt_status (*)(mvbt_tree*, const uint32_t*, const uint32_t*, uint64_t, const uint32_t*,
@@ -690,7 +741,7 @@ mvbt_status sorted_window(mvbt_tree* t, Add& add, batch_job* job, uint32_t* a, u
   uint32_t* totals = hist + (uint64_t)kRadix * kSortSegments;
   uint32_t* n_writes = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(job) + offsetof(batch_job, n_writes));
   uint32_t segs = kSortSegments;
-  const sorted_fns f = table_for(t->nw, true)->sorted();
+  const sorted_fns f = table_for(t->nw, true)->sorted(t->read_tile);
   tree_ptrs tp = t->ptrs();
   const uint32_t grid = (uint32_t)t->num_cus * 8;
   {
@@ -1474,6 +1525,22 @@ mvbt_status mvbt_set_worker_mode(mvbt_tree* t, uint32_t mode) {
     (void)hipDeviceSynchronize();
     destroy_graph(t);
     t->worker_mode = mode;
+  }
+  return MVBT_OK;
+}
+
+mvbt_status mvbt_set_read_tile(mvbt_tree* t, uint32_t lanes) {
+  if (mvbt_status rc = check_tree(t)) return rc;
+  const uint32_t nw = t->nw;
+  if (lanes != 0 && (lanes > nw || (lanes != 16 && lanes != 32 && lanes != 64) || nw / lanes > 4))
+    return fail(MVBT_ERR_INVALID_ARG, "read tile: 0 (the node width) or 16, 32, 64 lanes, at most the node width");
+  const uint32_t v = lanes == nw ? 0u : lanes;
+  std::lock_guard<std::mutex> lk(t->mu);
+  if (t->read_tile != v) {
+    device_guard g(t->device);
+    (void)hipDeviceSynchronize();
+    destroy_graph(t);  // the sorted form's graphs hold the read kernels
+    t->read_tile = v;
   }
   return MVBT_OK;
 }

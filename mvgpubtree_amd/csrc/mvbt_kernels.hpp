@@ -6,6 +6,7 @@
 // grid-stride so a launch can be sized to residency instead of to n.
 #pragma once
 #include "mvbt_tree.hpp"
+#include "mvbt_wide.hpp"
 
 namespace mvbt {
 
@@ -120,6 +121,55 @@ __global__ void __launch_bounds__(256)
         if (t.rank() == j) result = cnt;
       }
     }
+    if (active && counts) counts[i] = result;
+    if (t.any(active && result > L) && t.leader()) atomic_or32(&tv.st->error_flags, kErrRangeOverflow);
+  }
+}
+
+// Finds and ranges with read tiles of T < NW lanes (P = NW / T pairs per
+// lane, mvbt_wide.hpp), for launches in which no writer runs: the same
+// pass-based reader, with 64 / T walks per wave. Walks advanced per pass:
+// kWideFindGroup at the find kernel's 8 waves per SIMD, kWideRangeGroup in
+// the range kernel (whose occupancy is not pinned).
+template <int P>
+constexpr int kWideFindGroup = 4 / P;
+template <int P>
+constexpr int kWideRangeGroup = 8 / P;
+
+template <int NW, bool VER, int T>
+__global__ void __launch_bounds__(kFindBlock) __attribute__((amdgpu_waves_per_eu(8)))
+find_wide_kernel(tree_view<NW, VER> tv, const uint32_t* keys, uint32_t* values, uint64_t n, uint32_t ts) {
+  using rc_t = route_cache<NW, kFindCacheBytes>;
+  tile<T> t;
+  chunking<T> c;
+  __shared__ rc_t rc;
+  tv.build_route_cache(rc);
+  for (uint64_t base = c.first; base < n; base += c.stride) {
+    const uint64_t i = base + t.rank();
+    const bool active = i < n;
+    const uint32_t key = active ? keys[i] : 0u;
+    const uint32_t result =
+        read_chunk_wide<T, false, false, kWideFindGroup<NW / T>>(tv, key, 0u, ts, nullptr, 0u, t.ballot(active), t, &rc);
+    if (active) values[i] = result;
+  }
+}
+
+template <int NW, bool VER, int T>
+__global__ void __launch_bounds__(256)
+    range_wide_kernel(tree_view<NW, VER> tv, const uint32_t* lower, const uint32_t* upper, uint64_t* out,
+                      uint32_t* counts, uint32_t L, uint64_t n, uint32_t ts) {
+  using rc_t = route_cache<NW, kRangeCacheBytes>;
+  tile<T> t;
+  chunking<T> c;
+  __shared__ rc_t rc;
+  tv.build_route_cache(rc);
+  for (uint64_t base = c.first; base < n; base += c.stride) {
+    const uint64_t i = base + t.rank();
+    const bool active = i < n;
+    const uint32_t lo = active ? lower[i] : 0u;
+    const uint32_t hi = active ? upper[i] : 0u;
+    const uint32_t result = read_chunk_wide<T, true, false, kWideRangeGroup<NW / T>>(
+        tv, lo == 0 ? 1u : lo, hi, ts, out ? out + base * (uint64_t)L : nullptr, L, t.ballot(active), t, &rc);
     if (active && counts) counts[i] = result;
     if (t.any(active && result > L) && t.leader()) atomic_or32(&tv.st->error_flags, kErrRangeOverflow);
   }

@@ -155,6 +155,20 @@ def test_autotuner_sweeps_and_caches(dev, tmp_path):
     assert again == best
 
 
+def test_autotuner_sweeps_read_tiles_of_the_sorted_form(dev):
+    """Node width x read tile x waves/CU: the sorted form is timed with every
+    read tile up to its node width (16- and 32-lane tiles at 32-pair nodes,
+    16-, 32- and 64-lane tiles at 64-pair nodes)."""
+    from mvgpubtree_amd import autotune
+    w = wl.config4(seed=6, n_initial=100_000, num_batches=24, min_batch=32, max_batch=4096, range_capacity=256)
+    groups = [w.batches[0::2], w.batches[1::2]]
+    best, allr = autotune.tune_batched(w.initial_keys, w.initial_values, groups, w.range_capacity, dev,
+                                       node_widths=(16, 32, 64), waves=(8,), modes=(2,))
+    got = {(r.node_width, r.read_tile or r.node_width) for r in allr}
+    assert got == {(16, 16), (32, 16), (32, 32), (64, 16), (64, 32), (64, 64)}
+    assert best.mops == max(r.mops for r in allr)
+
+
 def hot_leaf_batches(seed, n_batches, hot):
     """Batches whose writes all fall on a few hot leaves: every leaf's writes
     span hundreds of batches (the sorted form's leaf groups then run past a

@@ -1,15 +1,16 @@
 """GPU parity at the configurations' stated sizes (the benchmark's own seeded
 workloads), bit-exact against the oracle:
 
-* config 2: 40M-key versioned tree (32-pair nodes, as measured), 10 batches of
-  500K range-32 queries at the pre-batch snapshot + 500K inserts: every count
-  and every pair, then the final key set;
+* config 2: 40M-key versioned tree (32-pair nodes read by 16-lane tiles, as
+  measured), 10 batches of 500K range-32 queries at the pre-batch snapshot +
+  500K inserts: every count and every pair, then the final key set;
 * config 3: 40M-key tree, 10 batches of 500K erases + 500K finds at the
   pre-batch snapshot, reclaiming after each batch: every find, the final key
   set, and reclaimed memory;
-* config 4: 40M-key tree, 4096 batches (log-uniform sizes in [32, 65536],
-  60/20/10/10 find/insert/erase/range) in one batched launch: every result
-  word and every range pair against the write-once replay;
+* config 4: 40M-key tree (32-pair nodes, 16-lane read passes, as measured),
+  4096 batches (log-uniform sizes in [32, 65536], 60/20/10/10
+  find/insert/erase/range) in one batched launch: every result word and
+  every range pair against the write-once replay;
 * config 5: 500M-key tree, requests of 1..16384 ops (80/10/10
   find/range/insert) through the served loop: every result and pair against
   the write-once replay of the same request stream.
@@ -51,6 +52,7 @@ def test_config2_full_size(dev):
     w = wl.config2(seed=2)  # 40M keys, 10 x 1M ops: the benchmark's workload
     L = w.range_capacity
     tree = mv.GpuVersionedBTree(pool_nodes=int((40_000_000 + 10 * 1_000_000) / 4) + (1 << 20), node_width=32)
+    tree.set_read_tile(16)  # as measured: 16-lane read tiles, two pairs per lane
     tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
     ref = oracle.SortedState(w.initial_keys, w.initial_values)
     note(f"config2: built in {time.time() - t0:.1f}s")
@@ -132,6 +134,7 @@ def test_config4_full_size(dev):
     tree = mv.GpuVersionedBTree(pool_nodes=int(40_000_000 / 4 * (16 / nw) + n_ops / 5), node_width=nw)
     tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
     tree.set_worker_mode(2)
+    tree.set_read_tile(16)  # as measured: 16-lane read passes over 32-pair nodes
     spare = tree.memory_stats().free_nodes
     assert spare < n_ops / 2
     e0 = len(tree.memory_series())
