@@ -802,13 +802,14 @@ __global__ void __launch_bounds__(kSortBlock) batch_split_kernel(batch_job* job)
 
 // 7. Routes every write to its leaf: sort key = leaf (cap_ for a key that is
 // not routed: reserved keys, which fail in the write pass), payload = the
-// write's position in w_rel; the parent goes to w_par.
-template <int NW>
+// write's position in w_rel; the parent goes to w_par. T: routing tile
+// width (T < NW: NW / T pairs per lane, mvbt_wide.hpp).
+template <int NW, int T = NW>
 __global__ void __launch_bounds__(256) batch_route_kernel(tree_ptrs tp, batch_job* job) {
   if (ld32_coherent(&job->bad)) return;
   const tree_view<NW, true> tv(tp);
-  tile<NW> t;
-  chunking<NW> c;
+  tile<T> t;
+  chunking<T> c;
   __shared__ route_cache<NW, kRangeCacheBytes> rc;
   tv.build_route_cache(rc);
   const uint32_t n = job->n_writes;
@@ -825,7 +826,10 @@ __global__ void __launch_bounds__(256) batch_route_kernel(tree_ptrs tp, batch_jo
     const uint64_t q = t.ballot(go);
     uint32_t st0 = kNone, sp0 = kNone, leaf = kNone, par = kNone;
     if (go) st0 = tv.route_lane(rc, key, sp0);
-    tv.template descend_chunk<false>(key, q, leaf, par, t, q, st0, sp0);
+    if constexpr (T < NW)
+      descend_chunk_wide<T, false, kWideRouteGroup<NW / T>>(tv, key, q, leaf, par, t, q, st0, sp0);
+    else
+      tv.template descend_chunk<false>(key, q, leaf, par, t, q, st0, sp0);
     if (act) {
       const bool routed = go && leaf < tv.cap_;
       hint[i] = routed ? leaf : tv.cap_;

@@ -281,7 +281,8 @@ struct sorted_fns {
 };
 template <int NW>
 sorted_fns sorted_fn(uint32_t read_tile) {
-  sorted_fns f{reinterpret_cast<void*>(&batch_route_kernel<NW>),
+  // routing: 16-lane tiles at 32- and 64-pair nodes, whatever the read tile
+  sorted_fns f{reinterpret_cast<void*>(&batch_route_kernel<NW, NW >= 32 ? 16 : NW>),
                reinterpret_cast<void*>(&batch_sorted_write_kernel<NW>),
                reinterpret_cast<void*>(&batch_sorted_find_kernel<NW>),
                reinterpret_cast<void*>(&batch_sorted_range_kernel<NW>)};

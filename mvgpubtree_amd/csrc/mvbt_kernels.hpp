@@ -135,6 +135,9 @@ template <int P>
 constexpr int kWideFindGroup = 4 / P;
 template <int P>
 constexpr int kWideRangeGroup = 8 / P;
+// Writers' routing with 16-lane sub-tiles: keys advanced per pass per sub-tile.
+template <int P>
+constexpr int kWideRouteGroup = 8 / P;
 
 template <int NW, bool VER, int T>
 __global__ void __launch_bounds__(kFindBlock) __attribute__((amdgpu_waves_per_eu(8)))
@@ -206,7 +209,16 @@ __global__ void __launch_bounds__(256) write_kernel(tree_view<NW, VER> tv, const
     if constexpr (NW == 32) {
       if (active && valid_key(key)) st0 = tv.route_lane(rc, key, sp0);
     }
-    tv.template descend_chunk<true>(key, q, leaf, par, t, q, st0, sp0);
+    if constexpr (NW >= 32) {
+      // routed by the tile's 16-lane halves (quarters), NW / 16 pairs per
+      // lane: lane j of sub-tile s is lane 16 s + j of the tile, so each
+      // key's route lands in its own lane; the write below uses all NW lanes
+      tile<16> h;
+      const uint64_t qh = h.ballot((q >> t.rank()) & 1);
+      descend_chunk_wide<16, true, kWideRouteGroup<NW / 16>>(tv, key, qh, leaf, par, h, qh, st0, sp0);
+    } else {
+      tv.template descend_chunk<true>(key, q, leaf, par, t, q, st0, sp0);
+    }
     // Leaves that take keys of this chunk are written together; the keys it
     // hands back (lock busy, split needed, leaf moved) go one at a time.
     // ctx.want is the chunk's exact count: refills for version copies take

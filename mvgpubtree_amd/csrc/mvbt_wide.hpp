@@ -39,6 +39,11 @@ struct wnode {
       q[2 * i + 1] = v.y;
     }
   }
+  // Coherent form for routing beside writers (8-byte agent-scope loads).
+  MVBT_DEV void load_coherent(const uint64_t* n, const tile_t& t) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) q[i] = ld_coherent(n + t.rank() * P + i);
+  }
   MVBT_DEV static bool data_pair(uint32_t g) { return g < kData; }
   // metadata: pair NW - 1 (last lane, last slot); version word: pair NW - 2
   MVBT_DEV uint64_t meta(const tile_t& t) const { return t.shfl64(q[P - 1], T - 1); }
@@ -214,6 +219,84 @@ MVBT_DEV uint32_t read_chunk_wide(const tree_view<NW, VER>& tv, uint32_t lo, uin
   if (RANGE) res = cnt;
   if (err && t.leader()) atomic_or32(&tv.st->error_flags, err);
   return res;
+}
+
+// Routing of a chunk with T-lane tiles over NW-pair nodes: the steps of
+// tree_view::descend_chunk (one tree level per pass for RG keys of the tile,
+// B-link right moves, `early` lanes stopping above the leaves with the leaf
+// taken from their parent's entry), node images spread over P pairs per
+// lane. CONC: coherent loads (writers of other tiles may run). On return
+// lane j holds the leaf (or the node where its walk stopped) for key j and
+// the node it was reached from (kNone after a right move), or kNone as leaf
+// when the walk left the pool (kErrCorrupt is flagged). A writer's tile of
+// NW lanes calls this through its NW / T sub-tiles, lane j of sub-tile s
+// being lane s * T + j of the writer's tile, so each key's route lands in
+// its own lane.
+template <int T, bool CONC, int RG, int NW, bool VER>
+MVBT_DEV void descend_chunk_wide(const tree_view<NW, VER>& tv, uint32_t key, uint64_t todo, uint32_t& leaf,
+                                 uint32_t& parent, const tile<T>& t, uint64_t early, uint32_t start,
+                                 uint32_t start_par) {
+  using wn_t = wnode<NW, VER, T>;
+  using node_t = node<NW, VER>;
+  const uint32_t r = t.rank();
+  uint32_t cur = tv.root_, par = kNone;
+  if (start < tv.cap_) {
+    cur = start;
+    par = start_par;
+  }
+  bool corrupt = false;
+  todo &= t.ballot(valid_key(key));
+  for (int base = 0; base < T; base += RG) {
+    uint64_t active = (todo >> base) & ((1ull << RG) - 1);
+    for (uint32_t pass = 0; active != 0; ++pass) {
+      if (pass >= kMaxReadSteps) {
+        corrupt = true;
+        if (r >= (uint32_t)base && r < (uint32_t)(base + RG) && ((active >> (r - base)) & 1)) cur = kNone;
+        break;
+      }
+      wn_t w[RG];
+#pragma unroll
+      for (int j = 0; j < RG; ++j) {
+        const uint32_t idx = t.shfl(cur, (uint32_t)(base + j));
+        if ((active >> j) & 1) {
+          if constexpr (CONC) w[j].load_coherent(tv.at(idx), t);
+          else w[j].load_weak(tv.at(idx), t);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RG; ++j) {
+        if (!((active >> j) & 1)) continue;
+        const uint32_t lj = (uint32_t)(base + j);
+        const uint32_t idx = t.shfl(cur, lj);  // not yet advanced
+        const wn_t& n = w[j];
+        const uint32_t k = t.shfl(key, lj);
+        const uint64_t m = n.meta(t);
+        uint32_t next = kNone, npar = kNone;
+        bool done = false, arrived = false;
+        if (k >= node_t::high_key(m)) {
+          next = node_t::sibling(m);
+        } else if (node_t::is_leaf(m)) {
+          done = true;
+        } else {
+          next = n.child_for(k, t);
+          npar = idx;
+          arrived = node_t::above_leaves(m) && ((early >> lj) & 1);
+        }
+        if (!done && next >= tv.cap_) {
+          corrupt = true;
+          done = true;
+          if (r == lj) cur = kNone;
+        } else if (!done && r == lj) {
+          cur = next;
+          par = npar;
+        }
+        if (done || arrived) active &= ~(1ull << j);
+      }
+    }
+  }
+  if (corrupt && t.leader()) atomic_or32(&tv.st->error_flags, kErrCorrupt);
+  leaf = cur;
+  parent = par;
 }
 
 }  // namespace mvbt
