@@ -305,7 +305,8 @@ def run_config2(args):
         del w4
         log(f"config4: batched {c4['batched_Mops']} vs plain {c4['plain_per_batch_Mops']} M ops/s")
         levels = [float(q) for q in args.extras_qps.split(",")]
-        c5 = served_vs_plain(args.served_keys, levels, args.extras_requests, args.step_ops, dev, args.served_mode)
+        c5 = served_vs_plain(args.served_keys, levels, args.extras_requests, args.step_ops, dev, args.served_mode,
+                             args.served_node_width, args.served_read_tile)
         c3 = run_config3(args)  # erase + find at the snapshot with reclamation, same key count and steps
         log(f"config3: {c3['Mops_incl_reclaim']} M ops/s, peak {c3['peak_live_MB']} MB")
         result["breakdown"]["erase_find_reclaim"] = {k: v for k, v in c3.items() if k != "timeline"} | {
@@ -529,7 +530,8 @@ def gpu_unique_keys(n: int, extra: int, seed: int, dev):
     return u32[:n].contiguous(), u32[n:].contiguous()
 
 
-def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev, mode: int = 3):
+def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev, mode: int = 3, nw: int = 16,
+                    read_tile: int = 0):
     """Served form (config 5): n-key tree resident in HBM; open-loop Poisson
     arrivals at each QPS level; request sizes log-uniform in [1, 16384] ops,
     80% find / 10% range (~32 keys) / 10% insert, one snapshot per request;
@@ -545,8 +547,9 @@ def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev, mode: int = 
     t0 = time.perf_counter()
     init, fresh = gpu_unique_keys(n, (len(levels) + 1) * n_req * 2000, seed=5, dev=dev)
     vals = init ^ torch.tensor(wl.VALUE_XOR - 2**32, dtype=torch.int32, device=dev)
-    pool = int(n / 6) + (8 << 20)
-    tree = mv.GpuVersionedBTree(pool_nodes=pool)
+    pool = int(n / 6 * (16 / nw)) + (8 << 20)
+    tree = mv.GpuVersionedBTree(pool_nodes=pool, node_width=nw)
+    tree.set_read_tile(read_tile if read_tile < nw else 0)
     tree.set_worker_mode(mode)
     torch.cuda.synchronize()
     tb = time.perf_counter()
@@ -594,7 +597,7 @@ def served_vs_plain(n: int, levels, n_req: int, step_ops: int, dev, mode: int = 
     tree.close()
     if flags:
         raise RuntimeError(f"device error flags 0x{flags:x}")
-    return {"keys": n, "build_Minserts_per_s": round(n / build_s / 1e6, 1), "requests_per_level": n_req,
+    return {"keys": n, "node_width": nw, "read_tile": read_tile or nw, "build_Minserts_per_s": round(n / build_s / 1e6, 1), "requests_per_level": n_req,
             "levels": out, "pool_live_GB": round(mem.live_bytes / 1e9, 2),
             "pool_peak_GB": round(mem.peak_bytes / 1e9, 2)}
 
@@ -603,7 +606,7 @@ def run_config5(args):
     import torch
     levels = [float(q) for q in args.qps.split(",")]
     return served_vs_plain(args.served_keys, levels, args.requests, args.step_ops, torch.device("cuda:0"),
-                           args.served_mode)
+                           args.served_mode, args.served_node_width, args.served_read_tile)
 
 
 def cpu_model() -> str:
@@ -687,6 +690,9 @@ def main():
     ap.add_argument("--c4-read-tile", type=int, default=16,
                     help="default run: config-4 read tile lanes (0: the node width; the autotuner's pick)")
     ap.add_argument("--served-mode", type=int, default=3, help="config 5: worker mode of the served steps (3: by step size)")
+    ap.add_argument("--served-node-width", type=int, default=16, help="config 5: node width of the served tree")
+    ap.add_argument("--served-read-tile", type=int, default=0,
+                    help="config 5: read tile lanes of the served tree (0: the node width)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
