@@ -356,8 +356,10 @@ def run_config3(args):
         return time.perf_counter() - t
 
     def make(n):
-        return mv.GpuVersionedBTree(pool_nodes=int(n / 4) + args.steps * args.ops + (1 << 20),
-                                    retire_capacity=1 << 23)
+        t = mv.GpuVersionedBTree(pool_nodes=int(n / 4 * 16 / args.c3_node_width) + args.steps * args.ops + (1 << 20),
+                                 node_width=args.c3_node_width, retire_capacity=1 << 23)
+        t.set_read_tile(args.c3_read_tile if args.c3_read_tile < args.c3_node_width else 0)
+        return t
 
     scratch = make(ww.initial_keys.size)
     scratch.insert(dev_tensor(ww.initial_keys, dev), dev_tensor(ww.initial_values, dev))
@@ -384,7 +386,7 @@ def run_config3(args):
     tree.close()
     if flags:
         raise RuntimeError(f"device error flags 0x{flags:x}")
-    return {"keys": args.keys, "batches": len(w.batches), "ops": ops,
+    return {"keys": args.keys, "node_width": args.c3_node_width, "batches": len(w.batches), "ops": ops,
             "Mops_incl_reclaim": round(ops / el / 1e6, 1),
             "peak_live_MB": round(st.peak_bytes / 1e6, 1), "final_live_MB": round(st.live_bytes / 1e6, 1),
             "reclaimed_MB": round(st.reclaimed_nodes * st.node_bytes / 1e6, 1), "timeline": timeline}
@@ -691,6 +693,8 @@ def main():
                     help="default run: config-4 read tile lanes (0: the node width; the autotuner's pick)")
     ap.add_argument("--served-mode", type=int, default=3, help="config 5: worker mode of the served steps (3: by step size)")
     ap.add_argument("--served-node-width", type=int, default=16, help="config 5: node width of the served tree")
+    ap.add_argument("--c3-node-width", type=int, default=16, help="config 3: node width")
+    ap.add_argument("--c3-read-tile", type=int, default=0, help="config 3: read tile lanes (0: the node width)")
     ap.add_argument("--served-read-tile", type=int, default=0,
                     help="config 5: read tile lanes of the served tree (0: the node width)")
     ap.add_argument("--profile-copy", action="store_true",
