@@ -258,7 +258,9 @@ def run_config2(args):
     achieved = per_launch_bytes / (kernel_ms[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"range_kernel<{args.node_width},versioned> ({n_rq} range-32 queries/launch)"
+                "kernel": (f"range_wide_kernel<{args.node_width},versioned,{args.read_tile}>"
+                           if 0 < args.read_tile < args.node_width else f"range_kernel<{args.node_width},versioned>")
+                + f" ({n_rq} range-32 queries/launch)"
                 if dom == "range32" else f"write_kernel<{args.node_width},versioned,insert> ({n_ins} inserts/launch)",
                 "avg_launch_ms": round(kernel_ms[dom], 4)}
     traffic = load_traffic(dom, args.node_width)
@@ -290,6 +292,9 @@ def run_config2(args):
             "build_Minserts_per_s": round(n_initial / build_s / 1e6, 2),
             "find_hits": hits, "find_node_width": args.find_node_width,
             "find_roofline_frac": round(q.size * BYTES["find"] / (find_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            # the whole step (inserts + ranges, snapshot and reclaim included) by algorithmic bytes
+            "step_roofline_frac": round((n_ins * BYTES["insert"] + n_rq * BYTES["range32"]) * args.steps / elapsed
+                                        / 1e9 / HBM_PEAK_GBS, 4),
             "live_nodes": mem.live_nodes, "peak_live_nodes": mem.peak_live_nodes,
             "reclaimed_nodes": mem.reclaimed_nodes},
     }
