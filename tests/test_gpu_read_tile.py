@@ -2,7 +2,8 @@
 lanes read an NW-pair node, NW / T pairs per lane. Finds and ranges at every
 snapshot of a random history, on versioned and plain trees, against the
 oracle and against the one-pair-per-lane reader; range counts above L; the
-sorted batched form's read passes; and the setter's argument checks."""
+sorted batched form's read passes, also across read-tile changes between
+graph launches; and the setter's argument checks."""
 from __future__ import annotations
 
 import numpy as np
@@ -139,3 +140,25 @@ def test_read_tile_setter_checks_its_argument(dev):
     t32.set_read_tile(16)
     t16.close()
     t32.close()
+
+
+def test_read_tile_change_rebuilds_the_sorted_graphs(dev):
+    """Calls through the launch graphs with one read tile, then another and
+    back: each change rebuilds the graphs (their read passes are
+    width-specific), and every call stays bit-exact."""
+    w = wl.config4(seed=71, n_initial=150_000, num_batches=90, min_batch=32, max_batch=4096, range_capacity=256)
+    tree = mv.GpuVersionedBTree(pool_nodes=1 << 19, node_width=32)
+    tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
+    tree.set_worker_mode(2)
+    keys, vals = w.initial_keys, w.initial_values
+    for i, (lanes, a, b) in enumerate([(0, 0, 30), (16, 30, 60), (0, 60, 90)]):
+        tree.set_read_tile(lanes)
+        r = drivers.BatchedRunner(w.batches[a:b], w.range_capacity, dev, seed=30 + i)
+        tree.reserve_batches(r.nops, b - a)
+        res = to_np(r.run(tree, use_graph=True))
+        want, wpairs, final = oracle.batch_expected(keys, vals, r.ops_host, r.offsets_host, w.range_capacity, r.slots)
+        assert np.array_equal(res, want), f"call {i} (tile {lanes})"
+        assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
+        keys, vals = final.keys, final.values
+    assert tree.error_flags() == 0, hex(tree.error_flags())
+    tree.close()
