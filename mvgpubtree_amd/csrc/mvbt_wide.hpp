@@ -7,10 +7,12 @@
 // their in-register shifts and split halves are lane moves of whole pairs.
 //
 // The layout in memory is the one of mvbt_node.hpp; pair g of a node sits in
-// lane g / P, slot g % P. Only reads of launches in which no writer runs use
-// these tiles (snapshot reads of the versioned tree, and plain-tree reads
-// without MVBT_CONCURRENT), through the pass-based reader below, which
-// follows tree_view::read_chunk step for step.
+// lane g / P, slot g % P. Two users: reads of launches in which no writer
+// runs (snapshot reads of the versioned tree, plain-tree reads without
+// MVBT_CONCURRENT), through the pass-based reader below, which follows
+// tree_view::read_chunk step for step; and the routing of writers
+// (descend_chunk_wide, coherent loads), whose hints are checked under the
+// leaf lock like those of tree_view::descend_chunk.
 #pragma once
 #include "mvbt_tree.hpp"
 
