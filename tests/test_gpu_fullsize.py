@@ -82,7 +82,11 @@ def test_config3_full_size(dev):
     tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
     ref = oracle.SortedState(w.initial_keys, w.initial_values)
     live0 = tree.memory_stats().live_nodes
-    note(f"config3: built in {time.time() - t0:.1f}s")
+    nk, height = tree.validate_tree_structure()
+    assert nk == w.initial_keys.size
+    # the height the algorithmic byte counts assume (16-pair nodes, 40M random keys)
+    assert height == 8, height
+    note(f"config3: built in {time.time() - t0:.1f}s, height {height}")
     for i, b in enumerate(w.batches):
         s = tree.take_snapshot()
         f = tree.find(to_dev(b.find_keys, dev), timestamp=s)
