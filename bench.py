@@ -697,10 +697,10 @@ def main():
     ap.add_argument("--c4-read-tile", type=int, default=16,
                     help="default run: config-4 read tile lanes (0: the node width; the autotuner's pick)")
     ap.add_argument("--served-mode", type=int, default=3, help="config 5: worker mode of the served steps (3: by step size)")
-    ap.add_argument("--served-node-width", type=int, default=16, help="config 5: node width of the served tree")
+    ap.add_argument("--served-node-width", type=int, default=32, help="config 5: node width of the served tree")
     ap.add_argument("--c3-node-width", type=int, default=16, help="config 3: node width")
     ap.add_argument("--c3-read-tile", type=int, default=0, help="config 3: read tile lanes (0: the node width)")
-    ap.add_argument("--served-read-tile", type=int, default=0,
+    ap.add_argument("--served-read-tile", type=int, default=16,
                     help="config 5: read tile lanes of the served tree (0: the node width)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")

@@ -11,9 +11,10 @@ workloads), bit-exact against the oracle:
   4096 batches (log-uniform sizes in [32, 65536], 60/20/10/10
   find/insert/erase/range) in one batched launch: every result word and
   every range pair against the write-once replay;
-* config 5: 500M-key tree, requests of 1..16384 ops (80/10/10
-  find/range/insert) through the served loop: every result and pair against
-  the write-once replay of the same request stream.
+* config 5: 500M-key tree (32-pair nodes, 16-lane read passes, as measured),
+  requests of 1..16384 ops (80/10/10 find/range/insert) through the served
+  loop: every result and pair against the write-once replay of the same
+  request stream.
 """
 from __future__ import annotations
 
@@ -186,8 +187,9 @@ def test_config5_full_size(dev):
     del init_sorted
     ks = ks_dev.cpu().numpy().view(np.uint32)
     note(f"config5: {n} keys generated ({time.time() - t0:.1f}s)")
-    tree = mv.GpuVersionedBTree(pool_nodes=int(n / 6) + (8 << 20))
+    tree = mv.GpuVersionedBTree(pool_nodes=int(n / 12) + (8 << 20), node_width=32)
     tree.set_worker_mode(3)  # by step size: small steps on the persistent worker, large ones sorted
+    tree.set_read_tile(16)  # as measured: 32-pair nodes, 16-lane read passes
     order = torch.randperm(n, generator=g, device=dev)
     k_ins = ks_dev[order]
     del order
