@@ -279,7 +279,13 @@ __global__ void __launch_bounds__(256)
       const uint32_t key = is_w ? wkeys[i] : 0u;
       const uint32_t val = (IR && is_w) ? wvals[i] : 0u;
       uint32_t leaf = kNone, par = kNone, res = 0;
-      tv.template descend_chunk<true>(key, q, leaf, par, t, q);
+      if constexpr (NW >= 32) {  // routed by 16-lane sub-tiles, as in write_kernel
+        tile<16> h;
+        const uint64_t qh = h.ballot((q >> t.rank()) & 1);
+        descend_chunk_wide<16, true, kWideRouteGroup<NW / 16>>(tv, key, qh, leaf, par, h, qh, kNone, kNone);
+      } else {
+        tv.template descend_chunk<true>(key, q, leaf, par, t, q);
+      }
       ctx.want = (uint32_t)__builtin_popcountll(q);
       q = tv.template write_chunk<OP>(key, val, q, leaf, res, ctx, t);
       while (q) {
