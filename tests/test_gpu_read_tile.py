@@ -3,7 +3,8 @@ lanes read an NW-pair node, NW / T pairs per lane. Finds and ranges at every
 snapshot of a random history, on versioned and plain trees, against the
 oracle and against the one-pair-per-lane reader; range counts above L; the
 sorted batched form's read passes, also across read-tile changes between
-graph launches; and the setter's argument checks."""
+graph launches; empty and one-leaf trees with degenerate ranges; and the
+setter's argument checks."""
 from __future__ import annotations
 
 import numpy as np
@@ -160,5 +161,35 @@ def test_read_tile_change_rebuilds_the_sorted_graphs(dev):
         assert np.array_equal(res, want), f"call {i} (tile {lanes})"
         assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * w.range_capacity], wpairs)
         keys, vals = final.keys, final.values
+    assert tree.error_flags() == 0, hex(tree.error_flags())
+    tree.close()
+
+
+@pytest.mark.parametrize("nw,lanes", TILES)
+def test_read_tile_edge_cases(dev, nw, lanes):
+    """An empty tree, then a one-leaf tree: finds miss or hit, ranges with
+    hi <= lo, lo = 0 and reserved bounds return what the oracle returns,
+    and chunks whose length is not a multiple of the tile width end cleanly."""
+    tree = mv.GpuVersionedBTree(pool_nodes=1 << 12, node_width=nw)
+    tree.set_read_tile(lanes)
+    ref = oracle.MVMap()
+    q = np.array([1, 2, 77, 0xFFFFFFFE], dtype=np.uint32)
+    lo = np.array([0, 1, 50, 100, 100, 0xFFFFFFF0, 5], dtype=np.uint32)
+    hi = np.array([10, 0xFFFFFFFF, 50, 99, 101, 0xFFFFFFFF, 0xFFFFFFFE], dtype=np.uint32)
+    for step in range(2):
+        s = tree.take_snapshot()
+        assert s == ref.take_snapshot()
+        for ts in (s, mv.LATEST):
+            assert np.array_equal(to_np(tree.find(to_dev(q, dev), timestamp=ts)), ref.find(q, ts))
+            p, n = tree.range_query(to_dev(lo, dev), to_dev(hi, dev), 8, timestamp=ts)
+            rp, rn = ref.range_query(lo, hi, 8, ts)
+            assert np.array_equal(to_np(n), rn), (step, ts)
+            assert np.array_equal(to_np(p, np.uint64), rp), (step, ts)
+        k = np.array([77, 100, 3, 0xFFFFFFFE, 1], dtype=np.uint32)  # fits one leaf
+        tree.insert(to_dev(k, dev), to_dev(wl.value_of(k), dev))
+        ref.insert(k, wl.value_of(k))
+    rng = np.random.default_rng(9)
+    odd = rng.integers(1, 200, size=37, dtype=np.uint64).astype(np.uint32)  # 37: not a multiple of any tile
+    assert np.array_equal(to_np(tree.find(to_dev(odd, dev))), ref.find(odd))
     assert tree.error_flags() == 0, hex(tree.error_flags())
     tree.close()
