@@ -251,6 +251,25 @@ def run_config2(args):
             raise RuntimeError(f"device error flags 0x{ftree.error_flags():x} (find tree)")
         ftree.close()
 
+    # The same batches through the reference's one-launch mixed call
+    # (concurrent_insert_range: every range of the batch at the call's
+    # snapshot beside its inserts) on a fresh tree of the same keys: the
+    # first batch warms up, the others are timed with their reclamations.
+    mtree = make_tree(n_initial)
+    mtree.insert(dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev))
+    for b in range(args.steps):
+        if b == 1:
+            torch.cuda.synchronize()
+            tm = time.perf_counter()
+        ik, iv, lo, hi = dbat[b]
+        mtree.concurrent_insert_range(ik, iv, lo, hi, L, result=out_pairs, counts=out_counts, stream=stream)
+        mtree.reclaim(stream)
+    torch.cuda.synchronize()
+    mixed_mops = sum(b.num_ops for b in batches[1:]) / (time.perf_counter() - tm) / 1e6 if args.steps > 1 else None
+    if mtree.error_flags():
+        raise RuntimeError(f"device error flags 0x{mtree.error_flags():x} (mixed call)")
+    mtree.close()
+
     # Dominant kernel of the step and its roofline.
     share = {"range32": kernel_ms["range32"], "insert": kernel_ms["insert"]}
     dom = max(share, key=share.get)
@@ -295,6 +314,7 @@ def run_config2(args):
             # the whole step (inserts + ranges, snapshot and reclaim included) by algorithmic bytes
             "step_roofline_frac": round((n_ins * BYTES["insert"] + n_rq * BYTES["range32"]) * args.steps / elapsed
                                         / 1e9 / HBM_PEAK_GBS, 4),
+            "mixed_insert_range_Mops": round(mixed_mops, 1) if mixed_mops else None,
             "live_nodes": mem.live_nodes, "peak_live_nodes": mem.peak_live_nodes,
             "reclaimed_nodes": mem.reclaimed_nodes},
     }

@@ -138,6 +138,10 @@ constexpr int kWideRangeGroup = 8 / P;
 // Writers' routing with 16-lane sub-tiles: keys advanced per pass per sub-tile.
 template <int P>
 constexpr int kWideRouteGroup = 8 / P;
+// Reads of the mixed calls (each walk also keeps one image for its check):
+// walks advanced per pass per sub-tile, within the write phase's registers.
+template <int P>
+constexpr int kWideConcGroup = 4 / P;
 
 template <int NW, bool VER, int T>
 __global__ void __launch_bounds__(kFindBlock) __attribute__((amdgpu_waves_per_eu(8)))
@@ -302,18 +306,29 @@ __global__ void __launch_bounds__(256)
       const uint32_t lo = is_r ? rlo[ri] : 0u;
       const uint32_t hi = (IR && is_r) ? rhi[ri] : 0u;
       uint32_t result = IR ? 0u : kEmpty;
-      while (q) {
-        const uint32_t j = (uint32_t)lowest_bit(q);
-        q &= q - 1;
-        uint32_t r;
-        if constexpr (IR) {
-          const uint64_t rj = base + j - nw;
-          uint64_t* slot = pairs ? pairs + rj * (uint64_t)L : nullptr;
-          r = tv.template range<true, false>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t);
-        } else {
-          r = tv.template find<true, false>(t.shfl(lo, j), ts, t);
+      if constexpr (VER && NW >= 32) {
+        // pass-based, by the tile's 16-lane sub-tiles (NW / 16 pairs per
+        // lane); a visible live leaf is confirmed by a later read of its
+        // version word before it is scanned
+        tile<16> hsub;
+        const uint64_t qh = hsub.ballot(is_r);
+        uint64_t* slot = (IR && is_r && pairs) ? pairs + ri * (uint64_t)L : nullptr;
+        result = read_chunk_wide_conc<16, IR, kWideConcGroup<NW / 16>>(tv, (IR && lo == 0) ? 1u : lo, hi, ts, slot,
+                                                                          L, qh, hsub);
+      } else {
+        while (q) {
+          const uint32_t j = (uint32_t)lowest_bit(q);
+          q &= q - 1;
+          uint32_t r;
+          if constexpr (IR) {
+            const uint64_t rj = base + j - nw;
+            uint64_t* slot = pairs ? pairs + rj * (uint64_t)L : nullptr;
+            r = tv.template range<true, false>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t);
+          } else {
+            r = tv.template find<true, false>(t.shfl(lo, j), ts, t);
+          }
+          if (t.rank() == j) result = r;
         }
-        if (t.rank() == j) result = r;
       }
       if (IR) {
         if (is_r && counts) counts[ri] = result;

@@ -223,6 +223,132 @@ MVBT_DEV uint32_t read_chunk_wide(const tree_view<NW, VER>& tv, uint32_t lo, uin
   return res;
 }
 
+// Pass-based snapshot reads beside writers, for the mixed calls, whose
+// writes are all stamped after the call's snapshot `ts`: a write to a leaf
+// visible at ts first stores a version copy, then publishes a new version
+// word, and only then changes the data. A live leaf visible at ts is
+// therefore used only once a later pass has read its version word unchanged:
+// the image of the visiting pass is kept (spread over the tile like any
+// image, one per walk of the group) and scanned when confirmed, while a
+// changed word sends the walk down the leaf's version chain, whose copies
+// are immutable once published. Loads are coherent. Lane j: key `lo` (find)
+// or range [lo, hi) with pairs to its own `out` (at most cap), as
+// read_chunk_wide with LANES; returns the value or kEmpty, or the count.
+template <int T, bool RANGE, int RG, int NW>
+MVBT_DEV uint32_t read_chunk_wide_conc(const tree_view<NW, true>& tv, uint32_t lo, uint32_t hi, uint32_t ts,
+                                       uint64_t* out, uint32_t cap, uint64_t todo, const tile<T>& t) {
+  using wn_t = wnode<NW, true, T>;
+  using node_t = node<NW, true>;
+  enum : uint32_t { kRoute = 0, kArrive = 1, kCheck = 2, kChain = 3 };
+  const uint32_t r = t.rank();
+  uint32_t cur = tv.root_, phase = kRoute, cnt = 0, seen_lo = 0, seen_hi = 0;
+  uint32_t res = RANGE ? 0u : kEmpty;
+  uint32_t err = 0;
+  todo &= t.ballot(valid_key(lo) && (!RANGE || hi > lo));
+  for (int base = 0; base < T; base += RG) {
+    uint64_t active = (todo >> base) & ((1ull << RG) - 1);
+    wn_t keep[RG];  // images waiting for their confirmation
+    for (uint32_t pass = 0; active != 0; ++pass) {
+      if (pass >= kMaxReadSteps) {
+        err |= kErrCorrupt;
+        break;
+      }
+      wn_t w[RG];
+#pragma unroll
+      for (int j = 0; j < RG; ++j) {
+        const uint32_t idx = t.shfl(cur, (uint32_t)(base + j));
+        if ((active >> j) & 1) w[j].load_coherent(tv.at(idx), t);
+      }
+#pragma unroll
+      for (int j = 0; j < RG; ++j) {
+        if (!((active >> j) & 1)) continue;
+        const uint32_t lj = (uint32_t)(base + j);
+        const uint32_t k = t.shfl(lo, lj);
+        const uint32_t h = RANGE ? t.shfl(hi, lj) : 0u;
+        uint64_t* oj = RANGE ? reinterpret_cast<uint64_t*>(t.shfl64(reinterpret_cast<uint64_t>(out), lj)) : nullptr;
+        uint32_t ph = t.shfl(phase, lj), c = t.shfl(cnt, lj), next = t.shfl(cur, lj);
+        const uint64_t ver = w[j].version(t);
+        bool done = false;
+        // scans an image visible at ts; continues to the sibling it names
+        auto visit = [&](const wn_t& n) {
+          const uint64_t m = n.meta(t);
+          if constexpr (RANGE) {
+            c += n.collect(k, h, oj, c, cap, t);
+            if (h <= node_t::high_key(m) || node_t::sibling(m) == kNoSibling) {
+              done = true;
+            } else {
+              next = node_t::sibling(m);
+              ph = kArrive;
+            }
+          } else {
+            const uint32_t v = n.lookup(k, t);
+            if (r == lj) res = v;
+            done = true;
+          }
+        };
+        bool fresh = true;  // w[j] is still to be looked at (a confirmed visit moves on)
+        if (ph == kCheck) {
+          if (ver == pack(t.shfl(seen_lo, lj), t.shfl(seen_hi, lj))) {
+            visit(keep[j]);
+            fresh = false;
+          } else {
+            ph = kArrive;  // written meanwhile: take this read as a fresh arrival
+          }
+        }
+        if (fresh && ph == kRoute) {
+          const uint64_t m = w[j].meta(t);
+          if (k >= node_t::high_key(m)) next = node_t::sibling(m);
+          else if (!node_t::is_leaf(m)) next = w[j].child_for(k, t);
+          else ph = kArrive;
+        }
+        if (!fresh) {
+        } else if (ph == kArrive) {
+          if (node_t::stamp(ver) > ts) {
+            next = node_t::older(ver);
+            ph = kChain;
+            if (next == kNone) {
+              err |= kErrVersionGap;
+              done = true;
+            }
+          } else {  // visible live leaf: read it again next pass before use
+            keep[j] = w[j];
+            ph = kCheck;
+            if (r == lj) {
+              seen_lo = lo32(ver);
+              seen_hi = hi32(ver);
+            }
+          }
+        } else if (ph == kChain) {
+          if (node_t::stamp(ver) > ts) {
+            next = node_t::older(ver);
+            if (next == kNone) {
+              err |= kErrVersionGap;
+              done = true;
+            }
+          } else {
+            visit(w[j]);  // a version copy: immutable once published
+          }
+        }
+        if (!done && next >= tv.cap_) {
+          err |= kErrCorrupt;
+          done = true;
+        }
+        if (r == lj) {
+          cnt = c;
+          if (!done) {
+            cur = next;
+            phase = ph;
+          }
+        }
+        if (done) active &= ~(1ull << j);
+      }
+    }
+  }
+  if (RANGE) res = cnt;
+  if (err && t.leader()) atomic_or32(&tv.st->error_flags, err);
+  return res;
+}
+
 // Routing of a chunk with T-lane tiles over NW-pair nodes: the steps of
 // tree_view::descend_chunk (one tree level per pass for RG keys of the tile,
 // B-link right moves, `early` lanes stopping above the leaves with the leaf
