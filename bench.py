@@ -411,8 +411,32 @@ def run_config3(args):
     tree.close()
     if flags:
         raise RuntimeError(f"device error flags 0x{flags:x}")
+
+    # The same batches through the reference's one-launch mixed call
+    # (concurrent_find_erase: every find at the call's snapshot beside its
+    # erases) on a fresh 32-pair tree of the same keys, reclaiming after each
+    # call; the first batch warms up.
+    mt = mv.GpuVersionedBTree(pool_nodes=int(args.keys / 8) + args.steps * args.ops + (1 << 20), node_width=32,
+                              retire_capacity=1 << 23)
+    mt.insert(dev_tensor(w.initial_keys, dev), dev_tensor(w.initial_values, dev))
+    dev_b = [(dev_tensor(b.find_keys, dev), dev_tensor(b.erase_keys, dev)) for b in w.batches]
+    fres = torch.empty(max(b.find_keys.size for b in w.batches), dtype=torch.int32, device=dev)
+    mixed = None
+    for b, (fk, ek) in enumerate(dev_b):
+        if b == 1:
+            torch.cuda.synchronize()
+            tm = time.perf_counter()
+        mt.concurrent_find_erase(fk, ek, find_results=fres[:fk.numel()])
+        mt.reclaim()
+    torch.cuda.synchronize()
+    if len(dev_b) > 1:
+        mixed = sum(b.num_ops for b in w.batches[1:]) / (time.perf_counter() - tm) / 1e6
+    if mt.error_flags():
+        raise RuntimeError(f"device error flags 0x{mt.error_flags():x} (mixed call)")
+    mt.close()
     return {"keys": args.keys, "node_width": args.c3_node_width, "batches": len(w.batches), "ops": ops,
             "Mops_incl_reclaim": round(ops / el / 1e6, 1),
+            "mixed_find_erase_Mops": round(mixed, 1) if mixed else None,
             "peak_live_MB": round(st.peak_bytes / 1e6, 1), "final_live_MB": round(st.live_bytes / 1e6, 1),
             "reclaimed_MB": round(st.reclaimed_nodes * st.node_bytes / 1e6, 1), "timeline": timeline}
 
