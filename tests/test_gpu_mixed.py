@@ -137,3 +137,45 @@ def test_plain_find_erase_within_reference_bounds(dev, nw):
     assert np.array_equal(tree.export(), post_export(final))
     assert tree.error_flags() == 0
     tree.close()
+
+
+@pytest.mark.parametrize("nw", WIDTHS)
+def test_versioned_mixed_calls_on_hot_leaves(dev, nw):
+    """Small trees, so that nearly every leaf a read scans is written by the
+    same call (version copies published mid-scan, leaves split under the
+    reads): five insert/range calls and five find/erase calls in a row,
+    every result word for word against the state before its call."""
+    rng = np.random.default_rng(300 + nw)
+    keys = wl.unique_keys(rng, 40_000)
+    tree = mv.GpuVersionedBTree(pool_nodes=1 << 16, node_width=nw)
+    ref = oracle.MVMap()
+    live = keys[:3000]
+    tree.insert(to_dev(live, dev), to_dev(wl.value_of(live), dev))
+    ref.insert(live, wl.value_of(live))
+    pos = 3000
+    L = 256
+    for call in range(5):
+        new = keys[pos:pos + 3000]
+        pos += 3000
+        lo = np.sort(rng.choice(np.concatenate([live, new]), 6000)).astype(np.uint32)
+        hi = np.minimum(lo.astype(np.uint64) + wl.span_for(40, live.size), 0xFFFFFFFF).astype(np.uint32)
+        ref.take_snapshot()
+        pairs, counts = tree.concurrent_insert_range(to_dev(new, dev), to_dev(wl.value_of(new), dev),
+                                                     to_dev(lo, dev), to_dev(hi, dev), L)
+        rp, rc = ref.range_query(lo, hi, L)
+        assert np.array_equal(to_np(counts)[: lo.size], rc), call
+        assert np.array_equal(to_np(pairs, np.uint64)[: lo.size * L], rp), call
+        ref.insert(new, wl.value_of(new))
+        live = np.concatenate([live, new])
+    for call in range(5):
+        era = live[rng.choice(live.size, 2000, replace=False)]
+        fk = np.concatenate([era, live[rng.integers(0, live.size, 4000)], keys[pos:pos + 1000]])
+        ref.take_snapshot()
+        got = to_np(tree.concurrent_find_erase(to_dev(fk, dev), to_dev(era, dev)))
+        assert np.array_equal(got[: fk.size], ref.find(fk)), call
+        ref.erase(era)
+        live = np.setdiff1d(live, era)
+        tree.reclaim()
+    assert np.array_equal(tree.export(), ref.export())
+    assert tree.error_flags() == 0, hex(tree.error_flags())
+    tree.close()
