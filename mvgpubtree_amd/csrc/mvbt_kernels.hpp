@@ -306,10 +306,10 @@ __global__ void __launch_bounds__(256)
       const uint32_t lo = is_r ? rlo[ri] : 0u;
       const uint32_t hi = (IR && is_r) ? rhi[ri] : 0u;
       uint32_t result = IR ? 0u : kEmpty;
-      if constexpr (VER && NW >= 32) {
+      if constexpr (VER) {
         // pass-based, by the tile's 16-lane sub-tiles (NW / 16 pairs per
-        // lane); a visible live leaf is confirmed by a later read of its
-        // version word before it is scanned
+        // lane; at 16-pair nodes the tile itself); a visible live leaf is
+        // confirmed by a later read of its version word before it is scanned
         tile<16> hsub;
         const uint64_t qh = hsub.ballot(is_r);
         uint64_t* slot = (IR && is_r && pairs) ? pairs + ri * (uint64_t)L : nullptr;

@@ -7,7 +7,8 @@
 // their in-register shifts and split halves are lane moves of whole pairs.
 //
 // The layout in memory is the one of mvbt_node.hpp; pair g of a node sits in
-// lane g / P, slot g % P. Two users: reads of launches in which no writer
+// lane g / P, slot g % P (P = 1, one pair per lane, serves the mixed calls'
+// reader at 16-pair nodes). Two users: reads of launches in which no writer
 // runs (snapshot reads of the versioned tree, plain-tree reads without
 // MVBT_CONCURRENT), through the pass-based reader below, which follows
 // tree_view::read_chunk step for step; and the routing of writers
@@ -23,7 +24,7 @@ typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 template <int NW, bool VER, int T>
 struct wnode {
   static_assert(T == 16 || T == 32 || T == 64, "read tile width must be 16, 32 or 64");
-  static_assert(NW % T == 0 && NW / T >= 2 && NW / T <= 4, "two or four pairs per lane");
+  static_assert(NW % T == 0 && NW / T >= 1 && NW / T <= 4, "one, two or four pairs per lane");
   static constexpr int P = NW / T;
   using base_t = node<NW, VER>;
   using tile_t = tile<T>;
@@ -33,12 +34,16 @@ struct wnode {
   uint64_t q[P];  // pairs rank * P .. rank * P + P - 1
 
   MVBT_DEV void load_weak(const uint64_t* n, const tile_t& t) {
-    const u64x2* s = reinterpret_cast<const u64x2*>(n + t.rank() * P);
+    if constexpr (P == 1) {
+      q[0] = n[t.rank()];
+    } else {
+      const u64x2* s = reinterpret_cast<const u64x2*>(n + t.rank() * P);
 #pragma unroll
-    for (int i = 0; i < P / 2; ++i) {
-      const u64x2 v = s[i];
-      q[2 * i] = v.x;
-      q[2 * i + 1] = v.y;
+      for (int i = 0; i < P / 2; ++i) {
+        const u64x2 v = s[i];
+        q[2 * i] = v.x;
+        q[2 * i + 1] = v.y;
+      }
     }
   }
   // Coherent form for routing beside writers (8-byte agent-scope loads).
