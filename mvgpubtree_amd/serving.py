@@ -175,6 +175,7 @@ class ContinuousBatchingServer:
             if not inflight and nxt < R and arrivals[nxt] > now:
                 time.sleep(min(arrivals[nxt] - now, 1e-3) if arrivals[nxt] - now > 2e-4 else 0)
         total = time.perf_counter() - t0
+        self.step_ops = step_ops  # operations of each step, in order
         lat = done_at - arrivals
         gpu = np.asarray(step_gpu_ms) if step_gpu_ms else np.zeros(1)
         return {"requests": R, "ops": self.reqs.num_ops, "steps": steps,

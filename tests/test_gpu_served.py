@@ -18,13 +18,14 @@ from conftest import to_dev, to_np
 pytestmark = pytest.mark.gpu
 
 
-def setup(dev, seed, n_initial=100_000, n_requests=300, qps=1e6, max_ops=2048, pool=1 << 18):
+def setup(dev, seed, n_initial=100_000, n_requests=300, qps=1e6, max_ops=2048, pool=1 << 18, nw=16, read_tile=0):
     rng = np.random.default_rng(seed)
     keys = wl.unique_keys(rng, n_initial + 400_000)
     init, fresh = keys[:n_initial], keys[n_initial:]
     reqs = serving.make_requests(rng, n_requests, qps, np.sort(init), fresh, max_ops=max_ops,
                                  range_capacity=128)
-    tree = mv.GpuVersionedBTree(pool_nodes=pool)
+    tree = mv.GpuVersionedBTree(pool_nodes=pool, node_width=nw)
+    tree.set_read_tile(read_tile)
     tree.insert(to_dev(init, dev), to_dev(wl.value_of(init), dev))
     return init, reqs, tree
 
@@ -36,6 +37,27 @@ def test_served_results_equal_sequential_requests(dev, max_inflight, use_graph):
                                            max_inflight=max_inflight, use_graph=use_graph)
     summary, lat = srv.run()
     r = srv.runner
+    want, wpairs, final = oracle.batch_expected(init, wl.value_of(init), r.ops_host, r.offsets_host,
+                                                r.L, r.slots)
+    assert np.array_equal(to_np(r.results)[: r.nops], want)
+    assert np.array_equal(to_np(r.pairs, np.uint64)[: r.slots * r.L], wpairs)
+    assert summary["steps"] > 1 and np.all(lat >= 0)
+    assert tree.validate_tree_structure()[0] == len(final)
+    assert tree.error_flags() == 0
+    tree.close()
+
+
+def test_served_steps_of_both_forms_on_a_wide_tree(dev):
+    """The served configuration: 32-pair nodes read by 16-lane tiles, worker
+    mode by step size. Bursts of requests make steps both below and above
+    the size at which the sorted form takes over; every result equals the
+    requests run one at a time."""
+    init, reqs, tree = setup(dev, seed=9, n_requests=400, qps=2e7, max_ops=8192, pool=1 << 19, nw=32,
+                             read_tile=16)
+    srv = serving.ContinuousBatchingServer(tree, reqs, dev, max_step_ops=1 << 16, max_inflight=2)
+    summary, lat = srv.run()
+    r = srv.runner
+    assert min(srv.step_ops) < 16384 <= max(srv.step_ops)  # both forms ran
     want, wpairs, final = oracle.batch_expected(init, wl.value_of(init), r.ops_host, r.offsets_host,
                                                 r.L, r.slots)
     assert np.array_equal(to_np(r.results)[: r.nops], want)
