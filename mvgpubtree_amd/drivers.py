@@ -30,7 +30,7 @@ def driver():
     global _drv
     if _drv is None:
         from . import load_library
-        load_library()  # libmvbt.so first (the driver links it)
+        load_library()  # libmvbt.so first (libmvbt_driver.so links against it)
         if not os.path.exists(DRIVER_PATH):
             raise ImportError(f"{DRIVER_PATH} is not built; run __graft_entry__.build()")
         D = ctypes.CDLL(DRIVER_PATH)
