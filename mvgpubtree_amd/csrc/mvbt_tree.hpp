@@ -1331,7 +1331,11 @@ struct tree_view : tree_ptrs {
   // PLANNED: `planned` (per lane, equal for the lanes of one leaf) is the
   // version-copy slot reserved for the leaf's epoch ahead of the launch and
   // already entered in the retire ring, so no copy allocates or retires here
-  // (kNone: the pool had no slot for it).
+  // (kNone: the pool had no slot for it). PLANNED also takes no leaf locks:
+  // in the sorted form's write pass every leaf has one writing tile (the one
+  // that owns its group of sorted writes; a split's new right node holds
+  // only keys of that group) and no reader runs, while the per-key path and
+  // the parents it updates still lock as everywhere else.
   // The sorted form (PLANNED) takes batches of 8 leaves: its chunks hold few
   // leaves, and the smaller batch lifts its write pass to 4 waves per SIMD.
   template <int OP, bool PLANNED = false>
@@ -1357,7 +1361,8 @@ struct tree_view : tree_ptrs {
       }
       // 1. lock every leaf of the chunk at once (each leader lane its own leaf)
       uint32_t old = kLockBit;
-      if ((lead >> r) & 1) old = atomic_or32(lock_word(leaf), kLockBit);
+      if (PLANNED) old = 0;  // see above: these leaves have no other writer
+      else if ((lead >> r) & 1) old = atomic_or32(lock_word(leaf), kLockBit);
       const uint64_t locked = lead & t.ballot(!(old & kLockBit));
       for (uint64_t m = lead & ~locked; m; m &= m - 1)
         left |= todo & t.ballot(my_lead == (uint32_t)lowest_bit(m));
@@ -1390,7 +1395,7 @@ struct tree_view : tree_ptrs {
         // are complete: that wait also covers the reloads just issued
         if (prev) {
           order_release();
-          if ((prev >> r) & 1) atomic_and32(lock_word(leaf), ~kLockBit);
+          if (!PLANNED && ((prev >> r) & 1)) atomic_and32(lock_word(leaf), ~kLockBit);
         }
         if (!got) break;
         const int ng = __builtin_popcountll(got);
