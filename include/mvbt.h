@@ -303,7 +303,9 @@ typedef struct mvbt_op {
  * The tree clock advances by num_batches (also for a rejected call); *snapshot_base
  * (device pointer, may be NULL) receives base. The launch runs alone among this
  * handle's writers: it waits for writes issued earlier on other streams, and writes
- * issued later on other streams (other run_batches calls included) wait for it.
+ * issued later on other streams (other run_batches calls included) wait for it. User
+ * kernels that write through a device view (mvbt_device_view) must not overlap it
+ * either (its reads and its sorted-form leaf writes assume no other writer runs).
  * Versioned trees only; scratch for num_batches and num_ops is allocated on first use
  * and when a call needs more (synchronous then). */
 mvbt_status mvbt_run_batches(mvbt_tree* tree, const mvbt_op* ops, uint64_t num_ops,
