@@ -332,6 +332,7 @@ def run_config2(args):
         levels = [float(q) for q in args.extras_qps.split(",")]
         c5 = served_vs_plain(args.served_keys, levels, args.extras_requests, args.step_ops, dev, args.served_mode,
                              args.served_node_width, args.served_read_tile)
+        result["breakdown"]["plain_config1"] = run_config1(dev)
         c3 = run_config3(args)  # erase + find at the snapshot with reclamation, same key count and steps
         log(f"config3: {c3['Mops_incl_reclaim']} M ops/s, peak {c3['peak_live_MB']} MB")
         result["breakdown"]["erase_find_reclaim"] = {k: v for k, v in c3.items() if k != "timeline"} | {
@@ -347,6 +348,47 @@ def run_config2(args):
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline_config2(w, batches[0])
     return result
+
+
+def run_config1(dev, reps: int = 5):
+    """Config 1: plain (unversioned) tree, 1M unique keys inserted in one
+    launch, then 1M finds (half present); both launches timed with HIP
+    events on their stream (the insert on fresh trees, each find launch on
+    the built tree), results checked against the inserted values."""
+    import torch
+
+    import mvgpubtree_amd as mv
+    from mvgpubtree_amd import workloads as wl
+
+    w = wl.config1(seed=1)
+    dk, dv, dq = dev_tensor(w.keys, dev), dev_tensor(w.values, dev), dev_tensor(w.queries, dev)
+    out = torch.empty(dq.numel(), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    ins_ms, find_ms = [], []
+    for rep in range(reps + 1):
+        tree = mv.GpuBlinkTree(pool_nodes=1 << 19)
+        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a.record(stream)
+        tree.insert(dk, dv, stream=stream)
+        b.record(stream)
+        tree.find(dq, out, stream=stream)
+        c.record(stream)
+        torch.cuda.synchronize()
+        if rep:  # the first round loads the code
+            ins_ms.append(a.elapsed_time(b))
+            find_ms.append(b.elapsed_time(c))
+        flags = tree.error_flags()
+        tree.close()
+        if flags:
+            raise RuntimeError(f"device error flags 0x{flags:x} (config 1)")
+    got = out.cpu().numpy().view(np.uint32)
+    present = np.isin(w.queries, w.keys)
+    if not (np.array_equal(got[present], w.queries[present] ^ np.uint32(wl.VALUE_XOR))
+            and np.all(got[~present] == 0xFFFFFFFF)):
+        raise RuntimeError("config 1 finds disagree with the inserted keys")
+    n = w.keys.size
+    return {"keys": n, "insert_Mops": round(n / float(np.median(ins_ms)) / 1e3, 1),
+            "find_Mops": round(dq.numel() / float(np.median(find_ms)) / 1e3, 1)}
 
 
 def run_config3(args):

@@ -1035,6 +1035,9 @@ mvbt_status mvbt_destroy(mvbt_tree* t) {
   return MVBT_OK;
 }
 
+// First stage of a large insert into an empty tree (see mvbt_insert).
+constexpr uint64_t kFirstStage = 1ull << 14;
+
 mvbt_status mvbt_insert(mvbt_tree* t, const uint32_t* keys, const uint32_t* values, uint64_t n,
                         void* stream, uint32_t flags) {
   (void)flags;
@@ -1044,8 +1047,25 @@ mvbt_status mvbt_insert(mvbt_tree* t, const uint32_t* keys, const uint32_t* valu
   device_guard g(t->device);
   std::lock_guard<std::mutex> lk(t->mu);
   hipStream_t s = (hipStream_t)stream;
+  const bool was_fresh = t->fresh;
   if (mvbt_status rc = before_write(t, s)) return rc;
-  if (mvbt_status rc = table_for(t->nw, t->versioned)->insert(t, keys, values, n, s)) return rc;
+  const dispatch_table* tb = table_for(t->nw, t->versioned);
+  if (was_fresh && n > kFirstStage) {
+    // An empty tree takes a large batch in growing stages (kFirstStage keys,
+    // then four times as many as are in by then): in one launch the early
+    // keys all split the same few leaves and update the same parents, one
+    // lock holder at a time. The stages run in order on the stream, within
+    // the same epoch (no snapshot between them).
+    uint64_t done = 0, stage = kFirstStage;
+    while (done < n) {
+      const uint64_t m = std::min(stage, n - done);
+      if (mvbt_status rc = tb->insert(t, keys + done, values + done, m, s)) return rc;
+      done += m;
+      stage = 4 * done;
+    }
+  } else if (mvbt_status rc = tb->insert(t, keys, values, n, s)) {
+    return rc;
+  }
   return after_write(t, s);
 }
 

@@ -169,16 +169,19 @@ def test_config1_plain_full_size(dev):
 
 @pytest.mark.parametrize("kind", ["plain", "versioned"])
 def test_large_single_launch_build(dev, kind):
-    """8M random keys into an empty tree in one insert launch: the tree grows
-    from one leaf while the launch runs, so routing done early in a chunk goes
-    stale by millions of leaves; writes must still land (no step bound, no
-    error flag) and every key must be found with its value."""
+    """8M random keys into a one-key tree in one insert launch: the tree
+    grows from one leaf while the launch runs, so routing done early in a
+    chunk goes stale by millions of leaves; writes must still land (no step
+    bound, no error flag) and every key must be found with its value. (An
+    insert into a fresh tree runs in growing stages instead; the first key
+    goes in alone so that this one is a single launch.)"""
     rng = np.random.default_rng(21)
     keys = wl.unique_keys(rng, 8_000_000)
     vals = wl.value_of(keys)
     cls = mv.GpuBlinkTree if kind == "plain" else mv.GpuVersionedBTree
     tree = cls(pool_nodes=1 << 21)
-    tree.insert(to_dev(keys, dev), to_dev(vals, dev))
+    tree.insert(to_dev(keys[:1], dev), to_dev(vals[:1], dev))
+    tree.insert(to_dev(keys[1:], dev), to_dev(vals[1:], dev))
     sync()
     assert_clean(tree)
     q = np.concatenate([keys[::8], wl.absent_keys(rng, 500_000, np.sort(keys))])
