@@ -338,6 +338,9 @@ def run_config2(args):
         result["breakdown"]["erase_find_reclaim"] = {k: v for k, v in c3.items() if k != "timeline"} | {
             "epochs_recorded": len(c3["timeline"]) - 1,
             "last_epoch": c3["timeline"][-1]}
+        c6 = run_config6(args)  # the reclaimer's memory timeline at the reference's Figure-6 shape
+        result["breakdown"]["reclaimer_memory"] = {k: v for k, v in c6.items() if k not in ("mixed", "per_batch")} | {
+            f: {k: v for k, v in c6[f].items() if k != "timeline"} for f in ("mixed", "per_batch")}
         result["breakdown"]["batched_vs_plain"] = c4
         result["breakdown"]["served_vs_plain"] = {
             "keys": c5["keys"], "requests_per_level": c5["requests_per_level"],
@@ -702,6 +705,88 @@ def run_config5(args):
                            args.served_mode, args.served_node_width, args.served_read_tile)
 
 
+def run_config6(args):
+    """The reclaimer's memory timeline at the shape of the reference's
+    concurrent_insert_range_reclaimer_bench.cu:50-153 and its archived
+    memory_45m_45m_50_16.csv: a 45M-key versioned tree of 16-pair nodes,
+    then 22.5M inserts of new keys and 22.5M range queries [k, k + 16) with
+    k drawn from the initial keys (rkg.hpp:50-73), L = 16. The reference
+    issues them as one concurrent_insert_range launch whose reclaimer
+    advanced 440 epochs; here they run as the same number of mixed calls
+    (concurrent_insert_range, ranges at the state before the call),
+    reclaiming after each, so each call closes one epoch of the timeline,
+    and again as per-batch launches (snapshot, ranges at it, inserts,
+    release, reclaim). Memory is counted as the reference counts it: live
+    nodes (retired but not yet freed included) and nodes ever allocated."""
+    import torch
+
+    import mvgpubtree_amd as mv
+
+    dev = torch.device("cuda:0")
+    n0, n1, calls, L, span = args.c6_keys, args.c6_keys // 2, args.c6_calls, 16, 16
+    init, new = gpu_unique_keys(n0, n1, seed=6, dev=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(66)
+    lo = init[torch.randperm(n0, generator=g, device=dev)[:n1]]
+    lo64 = lo.to(torch.int64) & 0xFFFFFFFF
+    hi64 = torch.clamp(lo64 + span, max=0xFFFFFFFF)
+    hi = torch.where(hi64 >= 2**31, hi64 - 2**32, hi64).to(torch.int32)
+    bounds = [round(i * n1 / calls) for i in range(calls + 1)]
+    pairs = torch.empty(max(b - a for a, b in zip(bounds, bounds[1:])) * L, dtype=torch.int64, device=dev)
+    counts = torch.empty(pairs.numel() // L, dtype=torch.int32, device=dev)
+
+    def one(form):
+        tree = mv.GpuVersionedBTree(pool_nodes=int((n0 + n1) / 4) + (1 << 22), node_width=16,
+                                    retire_capacity=1 << 23)
+        tree.insert(init, init)
+        torch.cuda.synchronize()
+        st0 = tree.memory_stats()
+        e0 = len(tree.memory_series())
+        t = time.perf_counter()
+        for a, b in zip(bounds, bounds[1:]):
+            m = b - a
+            if form == "mixed":
+                tree.concurrent_insert_range(new[a:b], new[a:b], lo[a:b], hi[a:b], L, result=pairs[:m * L],
+                                             counts=counts[:m])
+            else:
+                s = tree.take_snapshot()
+                tree.range_query(lo[a:b], hi[a:b], L, result=pairs[:m * L], counts=counts[:m], timestamp=s)
+                tree.insert(new[a:b], new[a:b])
+                tree.release_snapshot(s)
+            tree.reclaim()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        series = tree.memory_series()[e0:]
+        st = tree.memory_stats()
+        flags = tree.error_flags() & ~mv.FLAG_RANGE_OVERFLOW
+        tree.close()
+        if flags:
+            raise RuntimeError(f"device error flags 0x{flags:x} ({form})")
+        gib = st0.node_bytes / 2**30
+        live = [st0.live_nodes] + [int(r.live_nodes) for r in series]
+        total = [st0.live_nodes] + [int(r.allocated_total) for r in series]
+        rows = [{"epoch": i, "live_nodes": live[i], "allocated_total": total[i], "ratio": round(live[i] / total[i], 4),
+                 "GiB": round(live[i] * gib, 4)} for i in range(0, len(live), max(1, (len(live) - 1) // 10))]
+        if rows[-1]["epoch"] != len(live) - 1:
+            i = len(live) - 1
+            rows.append({"epoch": i, "live_nodes": live[i], "allocated_total": total[i],
+                         "ratio": round(live[i] / total[i], 4), "GiB": round(live[i] * gib, 4)})
+        return {"Mops": round(2 * n1 / el / 1e6, 1), "epochs": len(series), "node_bytes": st0.node_bytes,
+                "start_GiB": round(st0.live_nodes * gib, 4), "start_nodes": st0.live_nodes,
+                "peak_GiB": round(max(live) * gib, 4), "peak_nodes": max(live),
+                "peak_live_before_freeing_GiB": round(st.peak_live_nodes * gib, 4),
+                "end_nodes": live[-1], "allocated_total": total[-1], "end_ratio": round(live[-1] / total[-1], 4),
+                "timeline": rows}
+
+    out = {"initial_keys": n0, "inserts": n1, "ranges": n1, "range_span": span, "range_capacity": L,
+           "node_width": 16, "calls": calls}
+    for form in ("mixed", "per_batch"):
+        out[form] = one(form)
+        log(f"config6 {form}: start {out[form]['start_GiB']} GiB, peak {out[form]['peak_GiB']} GiB, "
+            f"live/allocated {out[form]['end_ratio']}")
+    return out
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -755,10 +840,11 @@ def main():
                     help="config 2: node width of the tree the find breakdown runs on (default: the "
                          "measured tree itself)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6),
                     help="2: headline mixed insert/range batches; 3: erase/find batches with epoch "
                          "reclamation and the memory timeline; 4: batched form vs plain; "
-                         "5: served form vs plain per-request path")
+                         "5: served form vs plain per-request path; 6: the reclaimer's memory timeline "
+                         "(45M keys, then 22.5M inserts + 22.5M ranges of span 16)")
     ap.add_argument("--served-keys", type=int, default=500_000_000, help="config 5: tree size")
     ap.add_argument("--qps", default="2000,8000,32000,128000", help="config 5: QPS steps")
     ap.add_argument("--requests", type=int, default=2000, help="config 5: requests per QPS step")
@@ -788,6 +874,9 @@ def main():
     ap.add_argument("--c3-read-tile", type=int, default=0, help="config 3: read tile lanes (0: the node width)")
     ap.add_argument("--served-read-tile", type=int, default=16,
                     help="config 5: read tile lanes of the served tree (0: the node width)")
+    ap.add_argument("--c6-keys", type=int, default=45_000_000,
+                    help="config 6: initial tree size (half as many inserts and as many ranges follow)")
+    ap.add_argument("--c6-calls", type=int, default=440, help="config 6: calls (one reclamation epoch each)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -806,6 +895,9 @@ def main():
         return
     if args.config == 5:
         print(json.dumps({"config5": run_config5(args)}), flush=True)
+        return
+    if args.config == 6:
+        print(json.dumps({"config6": run_config6(args)}), flush=True)
         return
     result = run_config2(args)
     print(json.dumps(result), flush=True)

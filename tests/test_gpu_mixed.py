@@ -179,3 +179,50 @@ def test_versioned_mixed_calls_on_hot_leaves(dev, nw):
     assert np.array_equal(tree.export(), ref.export())
     assert tree.error_flags() == 0, hex(tree.error_flags())
     tree.close()
+
+
+def test_versioned_insert_range_calls_reclaiming_between_them(dev):
+    """The shape of the reference's reclaimer benchmark
+    (concurrent_insert_range_reclaimer_bench.cu:50-153), scaled down: a
+    16-pair tree of 120K keys, then 60K inserts and 60K ranges [k, k + 16)
+    with k drawn from the initial keys (rkg.hpp:50-73), L = 16, in 30 mixed
+    calls with a reclamation after each. Every call's ranges match the state
+    before it; each reclamation frees every copy the call retired (no
+    snapshot is held), one epoch per call lands in the timeline, live nodes
+    never shrink (inserts only), and nodes ever allocated = live + freed."""
+    rng = np.random.default_rng(606)
+    keys = wl.unique_keys(rng, 180_000)
+    init, new = keys[:120_000], keys[120_000:]
+    lo = init[rng.permutation(init.size)[: new.size]]
+    hi = np.minimum(lo.astype(np.uint64) + 16, 0xFFFFFFFF).astype(np.uint32)
+    tree = mv.GpuVersionedBTree(pool_nodes=1 << 17, node_width=16)
+    tree.insert(to_dev(init, dev), to_dev(wl.value_of(init), dev))
+    ref = oracle.MVMap()
+    ref.build_sorted(np.sort(init), wl.value_of(np.sort(init)))
+    L, calls = 16, 30
+    e0 = len(tree.memory_series())
+    live_prev = tree.memory_stats().live_nodes
+    cut = np.linspace(0, new.size, calls + 1).astype(int)
+    for c in range(calls):
+        a, b = cut[c], cut[c + 1]
+        ref.take_snapshot()
+        pairs, counts = tree.concurrent_insert_range(to_dev(new[a:b], dev), to_dev(wl.value_of(new[a:b]), dev),
+                                                     to_dev(lo[a:b], dev), to_dev(hi[a:b], dev), L)
+        rp, rc = ref.range_query(lo[a:b], hi[a:b], L)
+        assert np.array_equal(to_np(counts)[: b - a], rc), c
+        assert np.array_equal(to_np(pairs, np.uint64)[: (b - a) * L], rp), c
+        assert rc.min() >= 1  # k itself is live in every range
+        ref.insert(new[a:b], wl.value_of(new[a:b]))
+        tree.reclaim()
+        st = tree.memory_stats()
+        assert st.retired_pending == 0, c
+        assert st.live_nodes >= live_prev, c
+        live_prev = st.live_nodes
+    ser = tree.memory_series()[e0:]
+    assert len(ser) == calls
+    assert np.all(ser.allocated_total == ser.live_nodes + ser.reclaimed_total)
+    assert np.all(np.diff(ser.allocated_total.astype(np.int64)) > 0)
+    assert int(ser.reclaimed_total[-1]) > new.size // 2  # a version copy per touched leaf, freed
+    assert np.array_equal(tree.export(), ref.export())
+    assert tree.error_flags() == 0, hex(tree.error_flags())
+    tree.close()
