@@ -239,14 +239,17 @@ MVBT_DEV uint32_t read_chunk_wide(const tree_view<NW, VER>& tv, uint32_t lo, uin
 // are immutable once published. Loads are coherent. Lane j: key `lo` (find)
 // or range [lo, hi) with pairs to its own `out` (at most cap), as
 // read_chunk_wide with LANES; returns the value or kEmpty, or the count.
+// `start`: lane j's first node (an internal node from the launch's route
+// cache, which can only lie left of the key's path), or kNone for the root.
 template <int T, bool RANGE, int RG, int NW>
 MVBT_DEV uint32_t read_chunk_wide_conc(const tree_view<NW, true>& tv, uint32_t lo, uint32_t hi, uint32_t ts,
-                                       uint64_t* out, uint32_t cap, uint64_t todo, const tile<T>& t) {
+                                       uint64_t* out, uint32_t cap, uint64_t todo, const tile<T>& t,
+                                       uint32_t start = kNone) {
   using wn_t = wnode<NW, true, T>;
   using node_t = node<NW, true>;
   enum : uint32_t { kRoute = 0, kArrive = 1, kCheck = 2, kChain = 3 };
   const uint32_t r = t.rank();
-  uint32_t cur = tv.root_, phase = kRoute, cnt = 0, seen_lo = 0, seen_hi = 0;
+  uint32_t cur = start < tv.cap_ ? start : tv.root_, phase = kRoute, cnt = 0, seen_lo = 0, seen_hi = 0;
   uint32_t res = RANGE ? 0u : kEmpty;
   uint32_t err = 0;
   todo &= t.ballot(valid_key(lo) && (!RANGE || hi > lo));
