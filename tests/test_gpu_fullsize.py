@@ -11,6 +11,9 @@ workloads), bit-exact against the oracle:
   4096 batches (log-uniform sizes in [32, 65536], 60/20/10/10
   find/insert/erase/range) in one batched launch: every result word and
   every range pair against the write-once replay;
+* the config-2 and config-3 batches through the one-launch mixed calls
+  (concurrent_insert_range, concurrent_find_erase), 32-pair nodes: every
+  read of a call against the state before it;
 * config 5: 500M-key tree (32-pair nodes, 16-lane read passes, as measured),
   requests of 1..16384 ops (80/10/10 find/range/insert) through the served
   loop: every result and pair against the write-once replay of the same
@@ -104,6 +107,49 @@ def test_config3_full_size(dev):
     assert st.live_nodes <= live0 + 20_000  # version copies all freed; erases never merge leaves
     assert np.array_equal(tree.export(), packed(ref))
     assert tree.error_flags() == 0
+    tree.close()
+
+
+def test_mixed_calls_full_size(dev):
+    """The config-2 and config-3 batches at 40M keys through the one-launch
+    mixed calls (concurrent_insert_range, concurrent_find_erase) on 32-pair
+    trees, as measured: every read of a call against the state before it,
+    then the final key set. Four calls of each kind (the first ones of the
+    benchmark's workloads), reclaiming after each."""
+    t0 = time.time()
+    w = wl.config2(seed=2, num_batches=4)
+    L = w.range_capacity
+    tree = mv.GpuVersionedBTree(pool_nodes=int((40_000_000 + 4 * 1_000_000) / 4) + (1 << 20), node_width=32)
+    tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
+    ref = oracle.SortedState(w.initial_keys, w.initial_values)
+    for i, b in enumerate(w.batches):
+        p, n = tree.concurrent_insert_range(to_dev(b.insert_keys, dev), to_dev(b.insert_values, dev),
+                                            to_dev(b.range_lo, dev), to_dev(b.range_hi, dev), L)
+        tree.reclaim()
+        rp, rn = ref.range_query(b.range_lo, b.range_hi, L)
+        assert np.array_equal(to_np(n), rn), f"insert_range call {i}: counts"
+        assert np.array_equal(to_np(p, np.uint64), rp), f"insert_range call {i}: pairs"
+        ref.insert(b.insert_keys, b.insert_values)
+        note(f"mixed insert_range: call {i} ok ({time.time() - t0:.1f}s)")
+        del p, n, rp
+    assert np.array_equal(tree.export(), packed(ref))
+    assert tree.error_flags() == 0, hex(tree.error_flags())
+    tree.close()
+    del ref
+
+    w = wl.config3(seed=3, num_batches=4)
+    tree = mv.GpuVersionedBTree(pool_nodes=int(40_000_000 / 8) + 4 * 1_000_000 + (1 << 20), node_width=32)
+    tree.insert(to_dev(w.initial_keys, dev), to_dev(w.initial_values, dev))
+    ref = oracle.SortedState(w.initial_keys, w.initial_values)
+    for i, b in enumerate(w.batches):
+        f = tree.concurrent_find_erase(to_dev(b.find_keys, dev), to_dev(b.erase_keys, dev))
+        tree.reclaim()
+        want = ref.find(b.find_keys)
+        assert np.array_equal(to_np(f)[: b.find_keys.size], want), f"find_erase call {i}"
+        assert ref.erase(b.erase_keys) == b.erase_keys.size
+        note(f"mixed find_erase: call {i} ok ({time.time() - t0:.1f}s)")
+    assert np.array_equal(tree.export(), packed(ref))
+    assert tree.error_flags() == 0, hex(tree.error_flags())
     tree.close()
 
 
