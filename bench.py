@@ -338,6 +338,15 @@ def run_config2(args):
         result["breakdown"]["erase_find_reclaim"] = {k: v for k, v in c3.items() if k != "timeline"} | {
             "epochs_recorded": len(c3["timeline"]) - 1,
             "last_epoch": c3["timeline"][-1]}
+        sweep_args = argparse.Namespace(**vars(args))
+        sweep_args.c7_sizes = args.extras_sweep_sizes
+        c7 = run_config7(sweep_args)  # the reference's mixed-call sweeps, harmonic means over a subsample
+        result["breakdown"]["mixed_sweeps"] = {
+            "sizes_M": c7["sizes_M"], "node_width": c7["node_width"],
+            "insert_range": [{k: (v["hmean_Mops"] if isinstance(v, dict) else v) for k, v in r.items()}
+                             for r in c7["insert_range"]],
+            "find_erase": [{k: (v["hmean_Mops"] if isinstance(v, dict) else v) for k, v in r.items()}
+                           for r in c7["find_erase"]]}
         c6 = run_config6(args)  # the reclaimer's memory timeline at the reference's Figure-6 shape
         result["breakdown"]["reclaimer_memory"] = {k: v for k, v in c6.items() if k not in ("mixed", "per_batch")} | {
             f: {k: v for k, v in c6[f].items() if k != "timeline"} for f in ("mixed", "per_batch")}
@@ -787,6 +796,125 @@ def run_config6(args):
     return out
 
 
+def run_config7(args):
+    """The reference's mixed-call sweeps (scripts/bench_concurrent_insert_range.sh,
+    bench_concurrent_erase_find.sh; benchmarks/concurrent_insert_range_bench.cu,
+    concurrent_erase_find_bench.cu): for every configuration, one mixed call
+    of num_ops operations on a freshly built tree, num_ops = 1M, 2M, ... 45M,
+    rate = num_ops / the call's time (HIP events around the call alone), and
+    the harmonic mean over the sizes, on the versioned and the plain tree.
+    Keys are a shuffled permutation of 1..N (rkg.hpp:77-97).
+
+    insert_range: initial 1M or 40M keys, update ratio 5/50/90 %, range
+    length 8 or 32: inserts of the keys not in the tree, ranges [k, k + len)
+    with k from the initial keys (all of them shuffled, then sampled with
+    replacement when more ranges are asked for, rkg.hpp:50-73), L = len.
+    find_erase: initial 45M keys, update ratio 5/50/90 %: erases of the first
+    keys generated, finds of the first keys of a shuffle of them (all present
+    before the call, rkg.hpp:37-47)."""
+    import torch
+
+    import mvgpubtree_amd as mv
+
+    dev = torch.device("cuda:0")
+    sizes = [int(float(x) * 1e6) for x in args.c7_sizes.split(",")] if args.c7_sizes != "all" else \
+        [m * 1_000_000 for m in range(1, 46)]
+    kinds = args.c7_trees.split(",")
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    s = torch.cuda.current_stream()
+
+    def as_u32(x):  # int64 key values < 2^31 -> int32 tensors of the same bits
+        return x.to(torch.int32)
+
+    def make(kind, n_keys, extra):
+        cls = mv.GpuVersionedBTree if kind == "versioned" else mv.GpuBlinkTree
+        pool = int(n_keys / 4) + int(extra) + (1 << 21)
+        return cls(pool_nodes=pool, node_width=args.c7_node_width, retire_capacity=1 << 24)
+
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        fn()
+        b.record(s)
+        b.synchronize()
+        return a.elapsed_time(b) * 1e-3
+
+    def hmean(v):
+        return len(v) / sum(1.0 / x for x in v)
+
+    for kind in kinds:  # loads both mixed kernels before the first timed call
+        w = as_u32(torch.randperm(200_000, generator=g, device=dev) + 1)
+        tree = make(kind, 200_000, 0)
+        tree.insert(w[:100_000], w[:100_000])
+        tree.concurrent_insert_range(w[100_000:], w[100_000:], w[:1000], w[:1000] + 8, 8)
+        tree.concurrent_find_erase(w[:1000], w[1000:2000])
+        torch.cuda.synchronize()
+        tree.close()
+    out = {"node_width": args.c7_node_width, "sizes_M": [x / 1e6 for x in sizes], "insert_range": [],
+           "find_erase": []}
+    max_ops = max(sizes)
+    pairs = torch.empty(int(max_ops * 0.95) * 32 + 32, dtype=torch.int64, device=dev)
+    counts = torch.empty(max_ops + 1, dtype=torch.int32, device=dev)
+    for init in (1_000_000, 40_000_000):
+        for rl in (8, 32):
+            for ratio in (0.05, 0.5, 0.9):
+                row = {"initial": init, "range_length": rl, "update_ratio": ratio}
+                for kind in kinds:
+                    rates = []
+                    for n_ops in sizes:
+                        n_ins = int(n_ops * ratio)
+                        n_rq = n_ops - n_ins
+                        perm = torch.randperm(init + n_ins, generator=g, device=dev) + 1
+                        keys = as_u32(perm)
+                        lo_src = keys[:init][torch.randperm(init, generator=g, device=dev)]
+                        if n_rq > init:
+                            extra = torch.randint(0, init, (n_rq - init,), generator=g, device=dev)
+                            lo_src = torch.cat([lo_src, keys[:init][extra]])
+                        lo = lo_src[:n_rq].contiguous()
+                        hi = lo + rl
+                        tree = make(kind, init + n_ins, 0)
+                        tree.insert(keys[:init], keys[:init])
+                        ins = keys[init:]
+                        el = timed(lambda: tree.concurrent_insert_range(ins, ins, lo, hi, rl, result=pairs[:n_rq * rl],
+                                                                        counts=counts[:n_rq]))
+                        flags = tree.error_flags() & ~mv.FLAG_RANGE_OVERFLOW
+                        tree.close()
+                        if flags:
+                            raise RuntimeError(f"device error flags 0x{flags:x} (insert_range {row} {kind} {n_ops})")
+                        rates.append(n_ops / el / 1e6)
+                        del perm, keys, lo_src, lo, hi, ins
+                    row[kind] = {"hmean_Mops": round(hmean(rates), 1), "Mops": [round(x, 1) for x in rates]}
+                    log(f"config7 insert_range init {init} range {rl} ratio {ratio} {kind}: "
+                        f"hmean {row[kind]['hmean_Mops']} M ops/s")
+                out["insert_range"].append(row)
+    init = 45_000_000
+    for ratio in (0.05, 0.5, 0.9):
+        row = {"initial": init, "update_ratio": ratio}
+        for kind in kinds:
+            rates = []
+            for n_ops in sizes:
+                n_up = int(n_ops * ratio)
+                n_q = n_ops - n_up
+                keys = as_u32(torch.randperm(init, generator=g, device=dev) + 1)
+                er = keys[:n_up].contiguous()
+                fk = keys[torch.randperm(init, generator=g, device=dev)[:n_q]].contiguous()
+                tree = make(kind, init, 0)
+                tree.insert(keys, keys)
+                res = counts[:max(n_q, 1)]
+                el = timed(lambda: tree.concurrent_find_erase(fk, er, find_results=res))
+                flags = tree.error_flags()
+                tree.close()
+                if flags:
+                    raise RuntimeError(f"device error flags 0x{flags:x} (find_erase {row} {kind} {n_ops})")
+                rates.append(n_ops / el / 1e6)
+                del keys, er, fk
+            row[kind] = {"hmean_Mops": round(hmean(rates), 1), "Mops": [round(x, 1) for x in rates]}
+            log(f"config7 find_erase ratio {ratio} {kind}: hmean {row[kind]['hmean_Mops']} M ops/s")
+        out["find_erase"].append(row)
+    return out
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -840,11 +968,12 @@ def main():
                     help="config 2: node width of the tree the find breakdown runs on (default: the "
                          "measured tree itself)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6),
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6, 7),
                     help="2: headline mixed insert/range batches; 3: erase/find batches with epoch "
                          "reclamation and the memory timeline; 4: batched form vs plain; "
                          "5: served form vs plain per-request path; 6: the reclaimer's memory timeline "
-                         "(45M keys, then 22.5M inserts + 22.5M ranges of span 16)")
+                         "(45M keys, then 22.5M inserts + 22.5M ranges of span 16); 7: the reference's "
+                         "mixed-call sweeps (one call of 1M..45M ops per fresh tree, harmonic means)")
     ap.add_argument("--served-keys", type=int, default=500_000_000, help="config 5: tree size")
     ap.add_argument("--qps", default="2000,8000,32000,128000", help="config 5: QPS steps")
     ap.add_argument("--requests", type=int, default=2000, help="config 5: requests per QPS step")
@@ -877,6 +1006,12 @@ def main():
     ap.add_argument("--c6-keys", type=int, default=45_000_000,
                     help="config 6: initial tree size (half as many inserts and as many ranges follow)")
     ap.add_argument("--c6-calls", type=int, default=440, help="config 6: calls (one reclamation epoch each)")
+    ap.add_argument("--c7-sizes", default="all",
+                    help="config 7: call sizes in M ops, comma-separated, or 'all' (1..45)")
+    ap.add_argument("--c7-trees", default="versioned,plain", help="config 7: tree kinds")
+    ap.add_argument("--extras-sweep-sizes", default="1,10,20,30,45",
+                    help="default run: config-7 call sizes (M ops) for the sweep harmonic means")
+    ap.add_argument("--c7-node-width", type=int, default=32, help="config 7: node width")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -898,6 +1033,9 @@ def main():
         return
     if args.config == 6:
         print(json.dumps({"config6": run_config6(args)}), flush=True)
+        return
+    if args.config == 7:
+        print(json.dumps({"config7": run_config7(args)}), flush=True)
         return
     result = run_config2(args)
     print(json.dumps(result), flush=True)
