@@ -316,6 +316,12 @@ __global__ void __launch_bounds__(256)
       const uint32_t lo = is_r ? rlo[ri] : 0u;
       const uint32_t hi = (IR && is_r) ? rhi[ri] : 0u;
       uint32_t result = IR ? 0u : kEmpty;
+      const uint32_t k0 = (IR && lo == 0) ? 1u : lo;
+      uint32_t start = kNone;  // each lane routes its key through the cached levels
+      {
+        uint32_t unused;
+        if (is_r && valid_key(k0)) start = tv.route_lane(rc, k0, unused);
+      }
       if constexpr (VER) {
         // pass-based, by the tile's 16-lane sub-tiles (NW / 16 pairs per
         // lane; at 16-pair nodes the tile itself); a visible live leaf is
@@ -323,12 +329,6 @@ __global__ void __launch_bounds__(256)
         tile<16> hsub;
         const uint64_t qh = hsub.ballot(is_r);
         uint64_t* slot = (IR && is_r && pairs) ? pairs + ri * (uint64_t)L : nullptr;
-        const uint32_t k0 = (IR && lo == 0) ? 1u : lo;
-        uint32_t start = kNone;
-        {
-          uint32_t unused;
-          if (is_r && valid_key(k0)) start = tv.route_lane(rc, k0, unused);
-        }
         result = read_chunk_wide_conc<16, IR, kWideConcGroup<NW / 16>>(tv, k0, hi, ts, slot, L, qh, hsub, start);
       } else {
         while (q) {
@@ -338,9 +338,9 @@ __global__ void __launch_bounds__(256)
           if constexpr (IR) {
             const uint64_t rj = base + j - nw;
             uint64_t* slot = pairs ? pairs + rj * (uint64_t)L : nullptr;
-            r = tv.template range<true, false>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t);
+            r = tv.template range<true, false>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t, t.shfl(start, j));
           } else {
-            r = tv.template find<true, false>(t.shfl(lo, j), ts, t);
+            r = tv.template find<true, false>(t.shfl(lo, j), ts, t, t.shfl(start, j));
           }
           if (t.rank() == j) result = r;
         }
