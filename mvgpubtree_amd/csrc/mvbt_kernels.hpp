@@ -331,16 +331,30 @@ __global__ void __launch_bounds__(256)
         uint64_t* slot = (IR && is_r && pairs) ? pairs + ri * (uint64_t)L : nullptr;
         result = read_chunk_wide_conc<16, IR, kWideConcGroup<NW / 16>>(tv, k0, hi, ts, slot, L, qh, hsub, start);
       } else {
+        // plain tree: the chunk's keys are routed together (RG keys a pass,
+        // each walk ending on a node read as a leaf), then every key reads
+        // its leaves under their locks, one key at a time
+        const bool go = is_r && valid_key(k0) && (!IR || hi > k0);
+        uint32_t leaf = kNone, unused_par;
+        if constexpr (NW >= 32) {
+          tile<16> h;
+          descend_chunk_wide<16, true, kWideRouteGroup<NW / 16>>(tv, k0, h.ballot(go), leaf, unused_par, h, 0ull,
+                                                                 start, kNone);
+        } else {
+          tv.template descend_chunk<true>(k0, t.ballot(go), leaf, unused_par, t, 0ull, start, kNone);
+        }
+        q = t.ballot(go);
         while (q) {
           const uint32_t j = (uint32_t)lowest_bit(q);
           q &= q - 1;
           uint32_t r;
+          node<NW, VER> n;  // loaded under the leaf's lock
           if constexpr (IR) {
             const uint64_t rj = base + j - nw;
             uint64_t* slot = pairs ? pairs + rj * (uint64_t)L : nullptr;
-            r = tv.template range<true, false>(t.shfl(lo, j), t.shfl(hi, j), ts, slot, L, t, t.shfl(start, j));
+            r = tv.template range_in<true, false>(t.shfl(k0, j), t.shfl(hi, j), ts, n, t.shfl(leaf, j), slot, L, t);
           } else {
-            r = tv.template find<true, false>(t.shfl(lo, j), ts, t, t.shfl(start, j));
+            r = tv.template find_in<true, false>(t.shfl(k0, j), ts, n, t.shfl(leaf, j), t);
           }
           if (t.rank() == j) result = r;
         }
