@@ -53,6 +53,42 @@ __global__ void init_kernel(tree_view<NW, VER> tv) {
   tv.at(leaf)[r] = lp;
 }
 
+// Finds and ranges with read tiles of T < NW lanes (P = NW / T pairs per
+// lane, mvbt_wide.hpp), for launches in which no writer runs: the same
+// pass-based reader, with 64 / T walks per wave. Walks advanced per pass:
+// kWideFindGroup at the find kernel's 8 waves per SIMD, kWideRangeGroup in
+// the range kernel (whose occupancy is not pinned).
+template <int P>
+constexpr int kWideFindGroup = 4 / P;
+template <int P>
+constexpr int kWideRangeGroup = 8 / P;
+// Writers' routing with 16-lane sub-tiles: keys advanced per pass per sub-tile.
+template <int P>
+constexpr int kWideRouteGroup = 8 / P;
+// Reads of the mixed calls (each walk also keeps one image for its check):
+// walks advanced per pass per sub-tile, within the write phase's registers.
+template <int P>
+constexpr int kWideConcGroup = 4 / P;
+
+// Routing of a chunk's ranges beside writers (MVBT_CONCURRENT): lane j's key
+// from `start` (its route-cache node, kNone: the root) to a node read as a
+// leaf, the keys of the chunk together (the writers' pass-based routing,
+// coherent loads; 16-lane sub-tiles at 32- and 64-pair nodes). Lanes not in
+// `go` get an unspecified node and must not use it.
+template <int NW, bool VER>
+MVBT_DEV uint32_t route_reads(const tree_view<NW, VER>& tv, uint32_t key, bool go, uint32_t start,
+                              const tile<NW>& t) {
+  uint32_t leaf = kNone, unused_par;
+  if constexpr (NW >= 32) {
+    tile<16> h;
+    descend_chunk_wide<16, true, kWideRouteGroup<NW / 16>>(tv, key, h.ballot(go), leaf, unused_par, h, 0ull, start,
+                                                           kNone);
+  } else {
+    tv.template descend_chunk<true>(key, t.ballot(go), leaf, unused_par, t, 0ull, start, kNone);
+  }
+  return leaf;
+}
+
 template <int NW, bool VER, bool CONC>
 __global__ void __launch_bounds__(kFindBlock) __attribute__((amdgpu_waves_per_eu(8)))
 find_kernel(tree_view<NW, VER> tv, const uint32_t* keys,
@@ -110,6 +146,29 @@ __global__ void __launch_bounds__(256)
       // no writer runs: every query of the chunk advances one node per pass
       result = tv.template read_chunk<true>(lo == 0 ? 1u : lo, hi, ts, out ? out + base * (uint64_t)L : nullptr,
                                             L, q, t, &rc);
+    } else if constexpr (NW == 32) {
+      // beside writers at 32-pair nodes: the chunk's ranges are routed
+      // together, then each range reads its leaves (under their locks at the
+      // latest state, validated against their version words at a snapshot).
+      // 11-13% faster here; at 16-pair nodes the routing's registers cost
+      // more than it saves (plain trees 25% slower), so those keep the walk
+      // below, range by range.
+      const uint32_t k0 = lo == 0 ? 1u : lo;
+      const bool go = active && valid_key(k0) && hi > k0;
+      uint32_t start = kNone, unused;
+      if (go) start = tv.route_lane(rc, k0, unused);
+      const uint32_t leaf = route_reads(tv, k0, go, start, t);
+      q = t.ballot(go);
+      while (q) {
+        uint32_t j = (uint32_t)lowest_bit(q);
+        q &= q - 1;
+        uint64_t* slot = out ? out + (base + j) * (uint64_t)L : nullptr;
+        const uint32_t idx = t.shfl(leaf, j);
+        node<NW, VER> n;
+        if (VER && ts != kLatest && idx < tv.cap_) tv.template load<true>(n, idx, t);
+        uint32_t cnt = tv.template range_in<true>(t.shfl(k0, j), t.shfl(hi, j), ts, n, idx, slot, L, t);
+        if (t.rank() == j) result = cnt;
+      }
     } else {
       uint32_t start = kNone, unused;
       if (active && valid_key(lo == 0 ? 1u : lo)) start = tv.route_lane(rc, lo == 0 ? 1u : lo, unused);
@@ -125,23 +184,6 @@ __global__ void __launch_bounds__(256)
     if (t.any(active && result > L) && t.leader()) atomic_or32(&tv.st->error_flags, kErrRangeOverflow);
   }
 }
-
-// Finds and ranges with read tiles of T < NW lanes (P = NW / T pairs per
-// lane, mvbt_wide.hpp), for launches in which no writer runs: the same
-// pass-based reader, with 64 / T walks per wave. Walks advanced per pass:
-// kWideFindGroup at the find kernel's 8 waves per SIMD, kWideRangeGroup in
-// the range kernel (whose occupancy is not pinned).
-template <int P>
-constexpr int kWideFindGroup = 4 / P;
-template <int P>
-constexpr int kWideRangeGroup = 8 / P;
-// Writers' routing with 16-lane sub-tiles: keys advanced per pass per sub-tile.
-template <int P>
-constexpr int kWideRouteGroup = 8 / P;
-// Reads of the mixed calls (each walk also keeps one image for its check):
-// walks advanced per pass per sub-tile, within the write phase's registers.
-template <int P>
-constexpr int kWideConcGroup = 4 / P;
 
 template <int NW, bool VER, int T>
 __global__ void __launch_bounds__(kFindBlock) __attribute__((amdgpu_waves_per_eu(8)))
