@@ -820,6 +820,7 @@ def run_config7(args):
     sizes = [int(float(x) * 1e6) for x in args.c7_sizes.split(",")] if args.c7_sizes != "all" else \
         [m * 1_000_000 for m in range(1, 46)]
     kinds = args.c7_trees.split(",")
+    widths = {"versioned": args.c7_node_width, "plain": args.c7_plain_node_width}
     g = torch.Generator(device=dev)
     g.manual_seed(7)
     s = torch.cuda.current_stream()
@@ -830,7 +831,7 @@ def run_config7(args):
     def make(kind, n_keys, extra):
         cls = mv.GpuVersionedBTree if kind == "versioned" else mv.GpuBlinkTree
         pool = int(n_keys / 4) + int(extra) + (1 << 21)
-        return cls(pool_nodes=pool, node_width=args.c7_node_width, retire_capacity=1 << 24)
+        return cls(pool_nodes=pool, node_width=widths[kind], retire_capacity=1 << 24)
 
     def timed(fn):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -851,7 +852,7 @@ def run_config7(args):
         tree.concurrent_find_erase(w[:1000], w[1000:2000])
         torch.cuda.synchronize()
         tree.close()
-    out = {"node_width": args.c7_node_width, "sizes_M": [x / 1e6 for x in sizes], "insert_range": [],
+    out = {"node_width": widths, "sizes_M": [x / 1e6 for x in sizes], "insert_range": [],
            "find_erase": []}
     max_ops = max(sizes)
     pairs = torch.empty(int(max_ops * 0.95) * 32 + 32, dtype=torch.int64, device=dev)
@@ -1011,7 +1012,9 @@ def main():
     ap.add_argument("--c7-trees", default="versioned,plain", help="config 7: tree kinds")
     ap.add_argument("--extras-sweep-sizes", default="1,10,20,30,45",
                     help="default run: config-7 call sizes (M ops) for the sweep harmonic means")
-    ap.add_argument("--c7-node-width", type=int, default=32, help="config 7: node width")
+    ap.add_argument("--c7-node-width", type=int, default=32, help="config 7: node width of the versioned tree")
+    ap.add_argument("--c7-plain-node-width", type=int, default=16,
+                    help="config 7: node width of the plain tree (16 pairs: its faster width for the mixed calls)")
     ap.add_argument("--profile-copy", action="store_true",
                     help="only run 1 GiB device copies (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
