@@ -70,6 +70,89 @@ constexpr int kWideRouteGroup = 8 / P;
 template <int P>
 constexpr int kWideConcGroup = 4 / P;
 
+// Plain-tree reads of a chunk beside writers, after their routing. A plain
+// leaf is rewritten in place, so it is read under its lock, and RG walks of
+// the tile advance together: a pass takes the locks of the walks' leaves
+// (test, then set, each walk by its own lane), loads the leaves it holds,
+// scans them and releases them. A walk whose leaf is busy tries again on the
+// next pass; no walk waits while it holds a lock. Lane j: key `key` (find)
+// or range [key, hi) with pairs to its own `out` (at most cap), from leaf
+// `leaf`; returns the value or kEmpty, or the count.
+constexpr int kLockedReadGroup = 4;
+template <bool RANGE, int NW>
+MVBT_DEV uint32_t read_locked_chunk(const tree_view<NW, false>& tv, uint32_t key, uint32_t hi, uint32_t leaf,
+                                    uint64_t todo, uint64_t* out, uint32_t cap, const tile<NW>& t) {
+  using node_t = node<NW, false>;
+  constexpr int RG = kLockedReadGroup;
+  const uint32_t r = t.rank();
+  uint32_t cur = leaf, cnt = 0, first = 1, res = RANGE ? 0u : kEmpty;
+  uint32_t err = 0;
+  todo &= t.ballot(leaf < tv.cap_);  // a walk whose routing left the pool has flagged it already
+  for (int base = 0; base < NW; base += RG) {
+    uint64_t act = (todo >> base) & ((1ull << RG) - 1);
+    for (uint32_t pass = 0; act != 0; ++pass) {
+      if (pass >= kMaxReadSteps) {
+        err |= kErrCorrupt;
+        break;
+      }
+      const bool mine = r >= (uint32_t)base && r < (uint32_t)(base + RG) && ((act >> (r - base)) & 1);
+      bool got = false;
+      if (mine) {
+        uint32_t old = ld32_coherent(tv.lock_word(cur));
+        if (!(old & kLockBit)) old = atomic_or32(tv.lock_word(cur), kLockBit);
+        got = !(old & kLockBit);
+      }
+      const uint64_t held = (t.ballot(got) >> base) & ((1ull << RG) - 1);
+      if (held == 0) {  // every leaf busy: back off before the next pass
+        for (uint32_t i = 0; i < 4; ++i) __builtin_amdgcn_s_sleep(2);
+        continue;
+      }
+      node_t n[RG];
+#pragma unroll
+      for (int j = 0; j < RG; ++j)
+        if ((held >> j) & 1) n[j].load(tv.at(t.shfl(cur, (uint32_t)(base + j))), t);
+      const uint32_t locked = cur;  // the leaf this lane's walk holds, if got
+#pragma unroll
+      for (int j = 0; j < RG; ++j) {
+        if (!((held >> j) & 1)) continue;
+        const uint32_t lj = (uint32_t)(base + j);
+        const uint64_t m = n[j].meta(t);
+        const uint32_t k = t.shfl(key, lj);
+        uint32_t next = kNone;
+        bool done = false;
+        if (t.shfl(first, lj) && k >= node_t::high_key(m)) {
+          next = node_t::sibling(m);  // split meanwhile: the key moved right
+        } else if constexpr (RANGE) {
+          const uint32_t h = t.shfl(hi, lj);
+          uint64_t* oj = reinterpret_cast<uint64_t*>(t.shfl64(reinterpret_cast<uint64_t>(out), lj));
+          const uint32_t c = t.shfl(cnt, lj) + n[j].collect(k, h, oj, t.shfl(cnt, lj), cap, t);
+          if (r == lj) {
+            cnt = c;
+            first = 0;
+          }
+          if (h <= node_t::high_key(m) || node_t::sibling(m) == kNoSibling) done = true;
+          else next = node_t::sibling(m);
+        } else {
+          const uint32_t v = n[j].lookup(k, t);
+          if (r == lj) res = v;
+          done = true;
+        }
+        if (!done && next >= tv.cap_) {
+          err |= kErrCorrupt;
+          done = true;
+        }
+        if (r == lj && !done) cur = next;
+        if (done) act &= ~(1ull << j);
+      }
+      order_release();  // the loads above have completed before a lock is released
+      if (got) atomic_and32(tv.lock_word(locked), ~kLockBit);
+    }
+  }
+  if (RANGE) res = cnt;
+  if (err && t.leader()) atomic_or32(&tv.st->error_flags, err);
+  return res;
+}
+
 // Routing of a chunk's ranges beside writers (MVBT_CONCURRENT): lane j's key
 // from `start` (its route-cache node, kNone: the root) to a node read as a
 // leaf, the keys of the chunk together (the writers' pass-based routing,
@@ -374,8 +457,8 @@ __global__ void __launch_bounds__(256)
         result = read_chunk_wide_conc<16, IR, kWideConcGroup<NW / 16>>(tv, k0, hi, ts, slot, L, qh, hsub, start);
       } else {
         // plain tree: the chunk's keys are routed together (RG keys a pass,
-        // each walk ending on a node read as a leaf), then every key reads
-        // its leaves under their locks, one key at a time
+        // each walk ending on a node read as a leaf), then they read their
+        // leaves under the leaves' locks, a group of walks per pass
         const bool go = is_r && valid_key(k0) && (!IR || hi > k0);
         uint32_t leaf = kNone, unused_par;
         if constexpr (NW >= 32) {
@@ -385,21 +468,8 @@ __global__ void __launch_bounds__(256)
         } else {
           tv.template descend_chunk<true>(k0, t.ballot(go), leaf, unused_par, t, 0ull, start, kNone);
         }
-        q = t.ballot(go);
-        while (q) {
-          const uint32_t j = (uint32_t)lowest_bit(q);
-          q &= q - 1;
-          uint32_t r;
-          node<NW, VER> n;  // loaded under the leaf's lock
-          if constexpr (IR) {
-            const uint64_t rj = base + j - nw;
-            uint64_t* slot = pairs ? pairs + rj * (uint64_t)L : nullptr;
-            r = tv.template range_in<true, false>(t.shfl(k0, j), t.shfl(hi, j), ts, n, t.shfl(leaf, j), slot, L, t);
-          } else {
-            r = tv.template find_in<true, false>(t.shfl(k0, j), ts, n, t.shfl(leaf, j), t);
-          }
-          if (t.rank() == j) result = r;
-        }
+        uint64_t* slot = (IR && is_r && pairs) ? pairs + ri * (uint64_t)L : nullptr;
+        result = read_locked_chunk<IR, NW>(tv, k0, hi, leaf, t.ballot(go), slot, L, t);
       }
       if (IR) {
         if (is_r && counts) counts[ri] = result;
