@@ -241,16 +241,21 @@ __global__ void __launch_bounds__(256)
       uint32_t start = kNone, unused;
       if (go) start = tv.route_lane(rc, k0, unused);
       const uint32_t leaf = route_reads(tv, k0, go, start, t);
-      q = t.ballot(go);
-      while (q) {
-        uint32_t j = (uint32_t)lowest_bit(q);
-        q &= q - 1;
-        uint64_t* slot = out ? out + (base + j) * (uint64_t)L : nullptr;
-        const uint32_t idx = t.shfl(leaf, j);
-        node<NW, VER> n;
-        if (VER && ts != kLatest && idx < tv.cap_) tv.template load<true>(n, idx, t);
-        uint32_t cnt = tv.template range_in<true>(t.shfl(k0, j), t.shfl(hi, j), ts, n, idx, slot, L, t);
-        if (t.rank() == j) result = cnt;
+      if constexpr (!VER) {  // plain leaves under their locks, four walks a pass
+        result = read_locked_chunk<true, NW>(tv, k0, hi, leaf, t.ballot(go), (go && out) ? out + i * (uint64_t)L : nullptr,
+                                             L, t);
+      } else {
+        q = t.ballot(go);
+        while (q) {
+          uint32_t j = (uint32_t)lowest_bit(q);
+          q &= q - 1;
+          uint64_t* slot = out ? out + (base + j) * (uint64_t)L : nullptr;
+          const uint32_t idx = t.shfl(leaf, j);
+          node<NW, VER> n;
+          if (ts != kLatest && idx < tv.cap_) tv.template load<true>(n, idx, t);
+          uint32_t cnt = tv.template range_in<true>(t.shfl(k0, j), t.shfl(hi, j), ts, n, idx, slot, L, t);
+          if (t.rank() == j) result = cnt;
+        }
       }
     } else {
       uint32_t start = kNone, unused;

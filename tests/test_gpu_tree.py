@@ -381,3 +381,52 @@ def test_concurrent_writers_on_two_streams_route_through_stale_caches(dev, kind)
     assert np.array_equal(to_np(tree.find(to_dev(q, dev))), ref.find(q))
     assert_clean(tree)
     tree.close()
+
+
+@pytest.mark.parametrize("nw", WIDTHS)
+def test_plain_concurrent_ranges_beside_inserts(dev, nw):
+    """Plain tree, MVBT_CONCURRENT ranges: with no writer running they equal
+    the oracle; on one stream while inserts run on another, every range
+    counts at least the keys present before the inserts and at most those
+    present after them (any linearization, as the reference's tolerant
+    validators allow, validation.hpp:266-369), and every pair it returns is a
+    key of the final state inside the range, with its value, in key order."""
+    import torch
+    rng = np.random.default_rng(77 + nw)
+    keys = wl.unique_keys(rng, 300000)
+    base, extra = keys[:200000], keys[200000:]
+    tree = mv.GpuBlinkTree(pool_nodes=1 << 18, node_width=nw)
+    tree.insert(to_dev(base, dev), to_dev(wl.value_of(base), dev))
+    before = oracle.SortedState(base, wl.value_of(base))
+    after = oracle.SortedState(keys, wl.value_of(keys))
+    lo = rng.integers(1, 0xFFFFFF00, size=60000, dtype=np.uint64).astype(np.uint32)
+    hi = np.minimum(lo.astype(np.uint64) + wl.span_for(32, 200000), 0xFFFFFFFF).astype(np.uint32)
+    dlo, dhi = to_dev(lo, dev), to_dev(hi, dev)
+    L = 128
+    p, n = tree.range_query(dlo, dhi, L, concurrent=True)
+    rp, rn = before.range_query(lo, hi, L)
+    assert np.array_equal(to_np(n), rn)
+    assert np.array_equal(to_np(p, np.uint64), rp)
+    sync()
+    sr, sw = torch.cuda.Stream(), torch.cuda.Stream()
+    with torch.cuda.stream(sw):
+        tree.insert(to_dev(extra, dev), to_dev(wl.value_of(extra), dev), stream=sw)
+    with torch.cuda.stream(sr):
+        p, n = tree.range_query(dlo, dhi, L, stream=sr, concurrent=True)
+    sync()
+    got_n = to_np(n).astype(np.int64)
+    _, n_after = after.range_query(lo, hi, L)
+    assert np.all(got_n >= rn) and np.all(got_n <= n_after)
+    assert got_n.max() <= L
+    pairs = to_np(p, np.uint64).reshape(lo.size, L)
+    used = np.arange(L)[None, :] < got_n[:, None]
+    k = (pairs & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    v = (pairs >> np.uint64(32)).astype(np.uint32)
+    ku = k[used]
+    assert np.all(np.isin(ku, keys))
+    assert np.all(v[used] == wl.value_of(ku))
+    assert np.all((k >= lo[:, None]) | ~used) and np.all((k < hi[:, None]) | ~used)
+    assert np.all((np.diff(k.astype(np.int64), axis=1) > 0) | ~used[:, 1:])
+    assert np.all(pairs[~used] == np.uint64(0xFFFFFFFFFFFFFFFF))
+    assert_clean(tree)
+    tree.close()
